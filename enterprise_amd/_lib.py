@@ -1,0 +1,135 @@
+"""ctypes binding of libent_bfs.so (the C ABI declared in include/ent_bfs.h and
+include/ent_graph.h). No torch types cross the boundary: plain pointers and
+sizes only.
+
+The library is loaded from enterprise_amd/lib/ (built in-tree by
+__graft_entry__.build()); a missing library raises instead of falling back to
+any other implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libent_bfs.so")
+
+ENT_OK = 0
+ENT_EINVAL = -1
+ENT_ENOMEM = -2
+ENT_EDEVICE = -3
+ENT_EINTERNAL = -4
+ENT_EIO = -5
+ENT_EBUSY = -6
+ENT_ECANCELED = -7
+
+DIR_AUTO, DIR_REFERENCE, DIR_TOPDOWN, DIR_BOTTOMUP = 0, 1, 2, 3
+LAUNCH_PERSISTENT, LAUNCH_LEVELS, LAUNCH_GRAPH, LAUNCH_STREAM = 0, 1, 2, 3
+
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+
+
+class CSR(C.Structure):
+    _fields_ = [("n", C.c_int64), ("m", C.c_int64),
+                ("beg_pos", i64p), ("adj", i32p)]
+
+
+class BfsConfig(C.Structure):
+    _fields_ = [(name, C.c_int32) for name in (
+        "direction", "launch", "alpha", "beta", "t_wave", "t_block", "t_grid",
+        "piece", "block_size", "blocks_per_cu", "bu_probe", "bu_chunk_words",
+        "graph_levels", "hub_cache")] + [("reserved", C.c_int32 * 2)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class Stats(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("td_levels", C.c_int32),
+                ("bu_levels", C.c_int32), ("phases", C.c_int32),
+                ("reached_vertices", C.c_int64), ("reached_edges", C.c_int64),
+                ("alg_bytes", C.c_int64), ("device_ms", C.c_double),
+                ("host_ms", C.c_double), ("bu_level_mask", C.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("k", C.c_int32), ("groups", C.c_int32), ("levels", C.c_int32),
+                ("reserved", C.c_int32),
+                ("reached_edges", C.c_int64), ("reached_vertices", C.c_int64),
+                ("alg_bytes", C.c_int64), ("device_ms", C.c_double),
+                ("host_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+# (name, restype, argtypes) for every exported function.
+SIGNATURES = [
+    ("ent_last_error", C.c_char_p, []),
+    ("ent_set_host_threads", None, [C.c_int32]),
+    ("ent_gen_rmat", C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double,
+                               C.c_double, C.c_uint64, C.c_int32, C.POINTER(CSR)]),
+    ("ent_gen_grid", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                               C.c_int32, C.POINTER(CSR)]),
+    ("ent_csr_from_edges", C.c_int, [C.c_int64, C.c_int64, i32p, i32p, C.POINTER(CSR)]),
+    ("ent_csr_from_arrays", C.c_int, [C.c_int64, C.c_int64, i64p, i32p, C.POINTER(CSR)]),
+    ("ent_csr_free", None, [C.POINTER(CSR)]),
+    ("ent_csr_check", C.c_int, [C.POINTER(CSR), C.c_int32]),
+    ("ent_csr_read_bin", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(CSR)]),
+    ("ent_csr_write_bin", C.c_int, [C.POINTER(CSR), C.c_char_p, C.c_char_p]),
+    ("ent_pick_roots", C.c_int, [C.POINTER(CSR), C.c_int32, C.c_uint64, i32p]),
+    ("ent_pick_roots_glibc", C.c_int, [C.POINTER(CSR), C.c_int32, i32p]),
+    ("ent_pick_roots_mixed", C.c_int, [C.POINTER(CSR), C.c_int32, C.c_int32,
+                                       C.c_uint64, i32p]),
+    ("ent_bfs_default_config", None, [C.POINTER(BfsConfig)]),
+    ("ent_graph_upload", C.c_int, [i64p, i32p, C.c_int64, C.c_int64, C.c_int,
+                                   C.POINTER(C.c_void_p)]),
+    ("ent_graph_free", C.c_int, [C.c_void_p]),
+    ("ent_graph_info", C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p]),
+    ("ent_set_config", C.c_int, [C.c_void_p, C.POINTER(BfsConfig)]),
+    ("ent_get_config", C.c_int, [C.c_void_p, C.POINTER(BfsConfig)]),
+    ("ent_bfs", C.c_int, [C.c_void_p, C.c_int32, i32p, C.POINTER(Stats)]),
+    ("ent_bfs_device", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(Stats)]),
+    ("ent_depth_device_ptr", C.c_void_p, [C.c_void_p]),
+    ("ent_copy_to_host", C.c_int, [C.c_void_p, C.c_void_p, i32p, C.c_int64]),
+    ("ent_bfs_gpu_coalescing_mem", C.c_int, [i32p, C.c_int32, i64p, i32p,
+                                             C.c_int64, C.c_int64, C.c_int32]),
+    ("ent_selftest_wave", C.c_int, [C.c_int]),
+]
+
+_lib = None
+
+
+class EntError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    """Loads libent_bfs.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        handle = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        for name, res, args in SIGNATURES:
+            fn = getattr(handle, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc):
+    if rc != ENT_OK:
+        msg = lib().ent_last_error()
+        raise EntError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,123 @@
+"""Device-resident graph handle and the single-source BFS drop-in.
+
+Mirrors the reference's entry point bfs_gpu_coalescing_mem
+(bfs_gpu_opt.cuh:293-447): a CSR is uploaded once (allocator.cuh:33-151) and
+each call runs one level-synchronous direction-optimizing BFS
+(bfs_tdbu_clfy_sort, bfs_gpu_opt.cuh:40-288) on the GPU through the C ABI of
+libent_bfs.so. Depth is int32 with -1 for unreached (the reference's
+unsigned char 255)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import BfsConfig, Stats, check, i32p, i64p, lib
+
+
+def default_config():
+    cfg = BfsConfig()
+    lib().ent_bfs_default_config(C.byref(cfg))
+    return cfg
+
+
+class DeviceGraph:
+    """A CSR graph resident in HBM plus its BFS workspace."""
+
+    def __init__(self, host_graph=None, beg_pos=None, adj=None, device=0):
+        if host_graph is not None:
+            beg_pos, adj = host_graph.beg_pos, host_graph.adj
+        b = np.ascontiguousarray(beg_pos, dtype=np.int64)
+        a = np.ascontiguousarray(adj, dtype=np.int32)
+        self.n = int(b.size - 1)
+        self.m = int(a.size)
+        self.device = device
+        h = C.c_void_p()
+        check(lib().ent_graph_upload(b.ctypes.data_as(i64p), a.ctypes.data_as(i32p),
+                                     self.n, self.m, int(device), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise RuntimeError("graph already freed")
+        return self._h
+
+    def info(self):
+        n, m, na, nb = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().ent_graph_info(self.handle, C.byref(n), C.byref(m), C.byref(na), C.byref(nb)))
+        return {"n": n.value, "m": m.value, "n_active": na.value, "device_bytes": nb.value}
+
+    @property
+    def config(self):
+        cfg = BfsConfig()
+        check(lib().ent_get_config(self.handle, C.byref(cfg)))
+        return cfg
+
+    def configure(self, **kw):
+        cfg = self.config
+        for k, v in kw.items():
+            if not hasattr(cfg, k):
+                raise KeyError(k)
+            setattr(cfg, k, int(v))
+        check(lib().ent_set_config(self.handle, C.byref(cfg)))
+        return cfg
+
+    def set_config(self, cfg):
+        check(lib().ent_set_config(self.handle, C.byref(cfg)))
+
+    def bfs(self, root, copy=True):
+        """One BFS; returns (depth int32[n] or None, stats dict)."""
+        st = Stats()
+        out = np.empty(self.n, dtype=np.int32) if copy else None
+        ptr = out.ctypes.data_as(i32p) if copy else None
+        check(lib().ent_bfs(self.handle, int(root), ptr, C.byref(st)))
+        return out, st.as_dict()
+
+    def bfs_device(self, root, depth_dev=None):
+        """One BFS writing to device memory (default: the handle's buffer)."""
+        st = Stats()
+        check(lib().ent_bfs_device(self.handle, int(root),
+                                   C.c_void_p(depth_dev) if depth_dev else None, C.byref(st)))
+        return st.as_dict()
+
+    def depth_device_ptr(self):
+        return lib().ent_depth_device_ptr(self.handle)
+
+    def copy_depth(self, src_dev=None, count=None):
+        count = self.n if count is None else count
+        out = np.empty(count, dtype=np.int32)
+        src = src_dev if src_dev is not None else self.depth_device_ptr()
+        check(lib().ent_copy_to_host(self.handle, C.c_void_p(src), out.ctypes.data_as(i32p), count))
+        return out
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().ent_graph_free(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def bfs_gpu_coalescing_mem(src_list, beg_pos, csr, gpu_id=0):
+    """The reference's entry point: per-root BFS with TEPS printout."""
+    s = np.ascontiguousarray(src_list, dtype=np.int32)
+    b = np.ascontiguousarray(beg_pos, dtype=np.int64)
+    a = np.ascontiguousarray(csr, dtype=np.int32)
+    return check(lib().ent_bfs_gpu_coalescing_mem(
+        s.ctypes.data_as(i32p), int(s.size), b.ctypes.data_as(i64p), a.ctypes.data_as(i32p),
+        int(b.size - 1), int(a.size), int(gpu_id)))
+
+
+def selftest_wave(device=0):
+    return check(lib().ent_selftest_wave(int(device)))

@@ -1,0 +1,75 @@
+// Wavefront (wave64) primitives for gfx950: DPP inclusive scan, ballot-based
+// compaction ranks, k-th set bit, scoped atomics and a bounded grid barrier.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ent {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Rank of this lane among the lanes whose bit is set in mask (mbcnt).
+__device__ __forceinline__ int mask_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+// Inclusive prefix sum across the 64 lanes with DPP: row_shr 1, 2, 4, 8 build
+// 16-lane row scans, then row_bcast:15 (rows 1 and 3) and row_bcast:31 (rows 2
+// and 3) carry row totals forward. All lanes must be active.
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+  return x;
+}
+
+__device__ __forceinline__ int wave_read(int x, int lane) {
+  return __builtin_amdgcn_readlane(x, lane);
+}
+
+// Sum over all 64 lanes (every lane receives the total).
+__device__ __forceinline__ int wave_sum(int x) { return wave_read(wave_incl_scan(x), 63); }
+
+// Position of the r-th (0-based) set bit of x; x must hold more than r bits.
+__device__ __forceinline__ int select_bit(uint64_t x, int r) {
+  int pos = 0;
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+    const int c = __popcll(x & ((1ull << s) - 1));
+    if (r >= c) {
+      r -= c;
+      x >>= s;
+      pos += s;
+    }
+  }
+  return pos;
+}
+
+// Orders LDS traffic between lanes of one wavefront (in-order LDS, but keep
+// the compiler from moving accesses across this point).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T>
+__device__ __forceinline__ T ld_agent(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// 100 MHz constant clock.
+__device__ __forceinline__ uint64_t rt_clock() { return wall_clock64(); }
+
+}  // namespace ent

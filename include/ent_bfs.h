@@ -1,0 +1,137 @@
+/*
+ * ent_bfs.h - C ABI of the MI355X (gfx950) direction-optimizing BFS.
+ *
+ * Drop-in for the reference's single-source GPU traversal
+ *   template<vertex_t,index_t> int bfs_gpu_coalescing_mem(vertex_t* src_list,
+ *       index_t* beg_pos, vertex_t* csr, index_t vert_count,
+ *       index_t edge_count, index_t gpu_id)          (bfs_gpu_opt.cuh:293-300)
+ * which allocates device memory and uploads the CSR (allocator.cuh:33-151),
+ * then runs one level-synchronous BFS per root (bfs_tdbu_clfy_sort,
+ * bfs_gpu_opt.cuh:40-288). Here those steps are separate entry points on an
+ * opaque graph handle, so a graph is uploaded once and queried many times.
+ *
+ * Depth convention: int32, root = 0, unreached = -1. The reference keeps an
+ * unsigned char depth with unreached = 255 (comm.h:69, :131); values 0..254
+ * agree and 255 maps to -1.
+ *
+ * Errors: every function returns ENT_OK (0) or a negative ENT_E* code (see
+ * ent_graph.h) and never exits the process; ent_last_error() gives the text.
+ * Threading: any thread may call; calls on one handle are serialised by a
+ * per-handle lock and run on the handle's own HIP stream.
+ */
+#ifndef ENT_BFS_H
+#define ENT_BFS_H
+
+#include <stdint.h>
+
+#include "ent_graph.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ent_graph ent_graph;
+
+/* Direction policy. */
+#define ENT_DIR_AUTO 0      /* Beamer alpha/beta switch, both ways */
+#define ENT_DIR_REFERENCE 1 /* reference rule: bottom-up once the frontier's
+                               degree sum reaches 0.3 * m at a level >= 1,
+                               never back (inspector.cuh:486-499) */
+#define ENT_DIR_TOPDOWN 2   /* top-down only */
+#define ENT_DIR_BOTTOMUP 3  /* bottom-up only */
+
+/* Launch form. */
+#define ENT_LAUNCH_PERSISTENT 0 /* one cooperative launch, grid barriers */
+#define ENT_LAUNCH_LEVELS 1     /* one launch per phase, host sync per level
+                                   (the reference's control flow) */
+#define ENT_LAUNCH_GRAPH 2      /* per-phase launches captured in a hipGraph,
+                                   replayed until the search ends */
+#define ENT_LAUNCH_STREAM 3     /* per-phase launches queued without host sync,
+                                   checked every graph_levels phases */
+
+typedef struct ent_bfs_config {
+  int32_t direction;       /* ENT_DIR_* */
+  int32_t launch;          /* ENT_LAUNCH_* */
+  int32_t alpha;           /* top-down -> bottom-up when m_f * alpha > m_u */
+  int32_t beta;            /* bottom-up -> top-down when n_f * beta < n_active
+                              and the frontier shrank */
+  int32_t t_wave;          /* degree >= t_wave: one wavefront per vertex */
+  int32_t t_block;         /* degree >= t_block: one block per vertex */
+  int32_t t_grid;          /* degree >= t_grid: vertex split into pieces of
+                              `piece` edges spread over the whole grid */
+  int32_t piece;           /* edges per grid-class piece (<= t_grid) */
+  int32_t block_size;      /* threads per block: 64..1024, multiple of 64 */
+  int32_t blocks_per_cu;   /* blocks per compute unit (0 = occupancy max) */
+  int32_t bu_probe;        /* neighbours one lane scans in bottom-up before
+                              its wavefront takes the vertex over */
+  int32_t bu_chunk_words;  /* 64-vertex bitmap words per wavefront chunk in
+                              the bottom-up sweep: 1..64 */
+  int32_t graph_levels;    /* phases per captured graph / queued batch */
+  int32_t hub_cache;       /* LDS hub-set entries (0 = off) */
+  int32_t reserved[2];
+} ent_bfs_config;
+
+typedef struct ent_stats {
+  int32_t levels;           /* max depth + 1 */
+  int32_t td_levels;        /* levels expanded top-down */
+  int32_t bu_levels;        /* levels expanded bottom-up */
+  int32_t phases;           /* kernel phases run (levels + inspections) */
+  int64_t reached_vertices; /* N_c */
+  int64_t reached_edges;    /* M_c = sum of degrees over reached vertices,
+                               the reference's TEPS numerator
+                               (validate.h:115-123) */
+  int64_t alg_bytes;        /* 8 M_c + 12 N_c + 4 n */
+  double device_ms;         /* device time of the search (HIP events) */
+  double host_ms;           /* wall time of the call */
+  uint64_t bu_level_mask;   /* bit L set: level L ran bottom-up (L < 64) */
+} ent_stats;
+
+/* Fills cfg with the library defaults. */
+void ent_bfs_default_config(ent_bfs_config *cfg);
+
+/* Uploads a CSR graph (host arrays, int64 offsets, int32 adjacency) to
+ * `device` and allocates the search workspace. Replaces allocator.cuh:33-151.
+ * n < 2^31; adjacency ids must lie in [0, n). */
+int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n,
+                     int64_t m, int device, ent_graph **out);
+int ent_graph_free(ent_graph *g);
+int ent_graph_info(const ent_graph *g, int64_t *n, int64_t *m,
+                   int64_t *n_active, int64_t *device_bytes);
+
+int ent_set_config(ent_graph *g, const ent_bfs_config *cfg);
+int ent_get_config(const ent_graph *g, ent_bfs_config *cfg);
+
+/* One BFS from root (bfs_tdbu_clfy_sort, bfs_gpu_opt.cuh:40-288). The depth
+ * array stays on the device; if depth_host != NULL it is also copied there
+ * (n int32, outside the timed search). stats may be NULL. */
+int ent_bfs(ent_graph *g, int32_t root, int32_t *depth_host, ent_stats *stats);
+
+/* Same, writing the n-element int32 depth array to device memory depth_dev
+ * (NULL = the handle's own buffer) and returning once the search is done. */
+int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev,
+                   ent_stats *stats);
+
+/* Device pointer of the handle's depth buffer (valid until ent_graph_free). */
+int32_t *ent_depth_device_ptr(ent_graph *g);
+
+/* Copies count int32 values from device memory to host memory. */
+int ent_copy_to_host(ent_graph *g, const int32_t *src_dev, int32_t *dst_host,
+                     int64_t count);
+
+/* Compatibility form of bfs_gpu_coalescing_mem (bfs_gpu_opt.cuh:293-447):
+ * uploads the graph, runs one BFS per root in src_list, and prints per-root
+ * and average TEPS as the reference does (roots reaching <= 2 levels are
+ * skipped, bfs_gpu_opt.cuh:412). Returns 0 or a negative code. */
+int ent_bfs_gpu_coalescing_mem(const int32_t *src_list, int32_t nsrc,
+                               const int64_t *beg_pos, const int32_t *csr,
+                               int64_t n, int64_t m, int32_t gpu_id);
+
+/* Device self-test of the wavefront primitives (DPP inclusive scan, ballot
+ * compaction, k-th set bit). Returns 0 when all lanes agree with the host. */
+int ent_selftest_wave(int device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ENT_BFS_H */

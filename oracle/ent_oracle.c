@@ -1,0 +1,219 @@
+/*
+ * ent_oracle.c - CPU oracle for the Enterprise BFS (test infrastructure only;
+ * never part of the product library).
+ *
+ * What each function restates from the reference (iHeartGraph/Enterprise):
+ *  - BFS semantics: depth[src] = 0 (expander.cuh:64); level L's frontier is
+ *    {v : depth[v] == L} (inspector.cuh:84); every unvisited neighbour of the
+ *    frontier gets L + 1 (expander.cuh:162-163, bottom-up form :519-523);
+ *    the search ends when a level discovers nothing (bfs_gpu_opt.cuh:193-202).
+ *    BFS hop distance is unique, so any correct BFS yields the same array.
+ *  - validate.h:31-103 (spot check) and its all-vertex generalisation.
+ *  - validate.h:106-127 (report: reached vertices and edges).
+ */
+#include "ent_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int32_t ent_oracle_bfs(int64_t n, const int64_t *beg_pos, const int32_t *adj,
+                       int32_t root, int32_t *depth) {
+  return ent_oracle_bfs_parents(n, beg_pos, adj, root, depth, NULL);
+}
+
+int32_t ent_oracle_bfs_parents(int64_t n, const int64_t *beg_pos,
+                               const int32_t *adj, int32_t root,
+                               int32_t *depth, int32_t *parent) {
+  if (n <= 0 || root < 0 || root >= n || !beg_pos || !depth) return -1;
+  int32_t *queue = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  if (!queue) return -1;
+  for (int64_t v = 0; v < n; ++v) depth[v] = -1;
+  if (parent)
+    for (int64_t v = 0; v < n; ++v) parent[v] = -1;
+  int64_t head = 0, tail = 0;
+  depth[root] = 0;
+  if (parent) parent[root] = root;
+  queue[tail++] = root;
+  int32_t maxd = 0;
+  while (head < tail) {
+    int32_t u = queue[head++];
+    int32_t du = depth[u];
+    for (int64_t i = beg_pos[u]; i < beg_pos[u + 1]; ++i) {
+      int32_t v = adj[i];
+      if (depth[v] < 0) {
+        depth[v] = du + 1;
+        if (parent) parent[v] = u;
+        if (du + 1 > maxd) maxd = du + 1;
+        queue[tail++] = v;
+      }
+    }
+  }
+  free(queue);
+  return maxd + 1;
+}
+
+int32_t ent_oracle_bfs_do(int64_t n, const int64_t *beg_pos, const int32_t *adj,
+                          int32_t root, int32_t *depth, int32_t threads) {
+  if (n <= 0 || root < 0 || root >= n || !beg_pos || !depth) return -1;
+#ifdef _OPENMP
+  if (threads < 1) threads = 1;
+#else
+  threads = 1;
+#endif
+  const int64_t m = beg_pos[n];
+  int32_t *cur = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  int32_t *nxt = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  if (!cur || !nxt) {
+    free(cur);
+    free(nxt);
+    return -1;
+  }
+  int64_t i;
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (i = 0; i < n; ++i) depth[i] = -1;
+  depth[root] = 0;
+  cur[0] = root;
+  int64_t ncur = 1;
+  int64_t mf = beg_pos[root + 1] - beg_pos[root];
+  int64_t mu = m - mf;
+  int bottom_up = 0;
+  int32_t level = 0;
+  const int64_t alpha = 15, beta = 18;
+  while (ncur > 0) {
+    if (!bottom_up && mf * alpha > mu) bottom_up = 1;
+    else if (bottom_up && ncur * beta < n) bottom_up = 0;
+    int64_t nnext = 0, mnext = 0;
+    if (bottom_up) {
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4096) reduction(+ : nnext, mnext)
+      for (i = 0; i < n; ++i) {
+        if (depth[i] >= 0) continue;
+        for (int64_t k = beg_pos[i]; k < beg_pos[i + 1]; ++k)
+          if (depth[adj[k]] == level) {
+            depth[i] = level + 1;
+            nnext += 1;
+            mnext += beg_pos[i + 1] - beg_pos[i];
+            break;
+          }
+      }
+      /* Rebuild the queue of the new frontier (needed if we switch back). */
+      int64_t w = 0;
+      for (i = 0; i < n; ++i)
+        if (depth[i] == level + 1) nxt[w++] = (int32_t)i;
+      nnext = w;
+    } else {
+      int64_t w = 0;
+#pragma omp parallel num_threads(threads) reduction(+ : mnext)
+      {
+#pragma omp for schedule(dynamic, 64)
+        for (i = 0; i < ncur; ++i) {
+          int32_t u = cur[i];
+          for (int64_t k = beg_pos[u]; k < beg_pos[u + 1]; ++k) {
+            int32_t v = adj[k];
+            if (depth[v] < 0 &&
+                __sync_bool_compare_and_swap(&depth[v], -1, level + 1)) {
+              int64_t slot = __sync_fetch_and_add(&w, 1);
+              nxt[slot] = v;
+              mnext += beg_pos[v + 1] - beg_pos[v];
+            }
+          }
+        }
+      }
+      nnext = w;
+    }
+    int32_t *t = cur;
+    cur = nxt;
+    nxt = t;
+    ncur = nnext;
+    mu -= mnext;
+    mf = mnext;
+    ++level;
+  }
+  free(cur);
+  free(nxt);
+  return level;
+}
+
+int32_t ent_oracle_validate(int64_t n, const int64_t *beg_pos, const int32_t *adj,
+                            int32_t root, const int32_t *depth) {
+  if (n <= 0 || root < 0 || root >= n || !beg_pos || !depth) return -8;
+  if (depth[root] != 0) return -5;
+  for (int64_t v = 0; v < n; ++v)
+    if (depth[v] < -1) return -7;
+  for (int64_t v = 0; v < n; ++v) {
+    const int32_t dv = depth[v];
+    int has_parent = (v == root);
+    for (int64_t i = beg_pos[v]; i < beg_pos[v + 1]; ++i) {
+      const int32_t du = depth[adj[i]];
+      if ((dv < 0) != (du < 0)) return -1;
+      if (dv < 0) continue;
+      if (du > dv + 1 || du < dv - 1) return -1;
+      if (du == dv - 1) has_parent = 1;
+    }
+    if (dv >= 0 && !has_parent) return -6;
+  }
+  return 0;
+}
+
+/* validate.h:57-99 for one base vertex. The reference's depth_t is unsigned
+ * char, so INFTY (255) compares as the largest depth; map -1 to INT32_MAX. */
+static int32_t spot_depth(const int32_t *depth, int32_t v) {
+  return depth[v] < 0 ? 0x7fffffff : depth[v];
+}
+
+int32_t ent_oracle_validate_spot(int64_t n, const int64_t *beg_pos, const int32_t *adj,
+                                 const int32_t *depth, int32_t base_v) {
+  if (n <= 0 || base_v < 0 || base_v >= n || depth[base_v] < 0) return -8;
+  const int64_t base = depth[base_v];
+  for (int64_t i = beg_pos[base_v]; i < beg_pos[base_v + 1]; ++i) {
+    const int32_t c = adj[i];
+    const int64_t cd = spot_depth(depth, c);
+    if (cd > base + 1 || cd < base - 1) return -1;
+    for (int64_t j = beg_pos[c]; j < beg_pos[c + 1]; ++j) {
+      const int64_t x = spot_depth(depth, adj[j]);
+      if (cd < base && x > base) return -2;
+      if (cd > base && x < base) return -3;
+      if (cd == base && (x > base + 1 || x < base - 1)) return -4;
+    }
+  }
+  return 0;
+}
+
+static int is_neighbour(const int64_t *beg_pos, const int32_t *adj, int32_t v, int32_t u) {
+  for (int64_t i = beg_pos[v]; i < beg_pos[v + 1]; ++i)
+    if (adj[i] == u) return 1;
+  return 0;
+}
+
+int32_t ent_oracle_validate_parents(int64_t n, const int64_t *beg_pos, const int32_t *adj,
+                                    int32_t root, const int32_t *depth,
+                                    const int32_t *parent) {
+  if (n <= 0 || root < 0 || root >= n || !depth || !parent) return -8;
+  if (parent[root] != root) return -9;
+  for (int64_t v = 0; v < n; ++v) {
+    if (v == root) continue;
+    const int32_t p = parent[v];
+    if (depth[v] < 0) {
+      if (p != -1) return -12;
+      continue;
+    }
+    if (p < 0 || p >= n) return -12;
+    if (!is_neighbour(beg_pos, adj, (int32_t)v, p)) return -10;
+    if (depth[p] != depth[v] - 1) return -11;
+  }
+  return 0;
+}
+
+void ent_oracle_report(int64_t n, const int64_t *beg_pos, const int32_t *depth,
+                       int64_t *reached_edges, int64_t *reached_vertices) {
+  int64_t e = 0, c = 0;
+  for (int64_t v = 0; v < n; ++v)
+    if (depth[v] >= 0) {
+      ++c;
+      e += beg_pos[v + 1] - beg_pos[v];
+    }
+  if (reached_edges) *reached_edges = e;
+  if (reached_vertices) *reached_vertices = c;
+}

@@ -1,0 +1,141 @@
+"""GPU parity of the single-source BFS (through the C ABI) against the CPU
+oracle: bit-exact int32 depth arrays and exact reached-vertex / reached-edge
+counts, for every direction policy and launch form."""
+import sys
+
+import numpy as np
+import pytest
+
+import graphs
+from enterprise_amd import DeviceGraph, HostGraph
+from enterprise_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+DIRS = {"auto": L.DIR_AUTO, "reference": L.DIR_REFERENCE, "topdown": L.DIR_TOPDOWN,
+        "bottomup": L.DIR_BOTTOMUP}
+LAUNCHES = {"stream": L.LAUNCH_STREAM, "levels": L.LAUNCH_LEVELS, "graph": L.LAUNCH_GRAPH,
+            "persistent": L.LAUNCH_PERSISTENT}
+
+# Thresholds small enough that a scale-16 R-MAT fills every class, including
+# grid-class pieces.
+TINY_CLASSES = dict(t_wave=4, t_block=24, t_grid=96, piece=64)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _check(oracle, g, dg, root, tag):
+    want = oracle.bfs(g.beg_pos, g.adj, root)
+    _say(f"[gpu] {tag} root={root}")
+    got, st = dg.bfs(root)
+    if not np.array_equal(got, want):
+        bad = np.flatnonzero(got != want)
+        raise AssertionError(f"{tag} root {root}: {bad.size} depth mismatches, first v={bad[0]} "
+                             f"gpu={got[bad[0]]} oracle={want[bad[0]]}")
+    edges, verts = oracle.report(g.beg_pos, want)
+    assert st["reached_edges"] == edges, (tag, st, edges)
+    assert st["reached_vertices"] == verts, (tag, st, verts)
+    assert st["levels"] == int(want.max()) + 1, (tag, st)
+    return st
+
+
+def test_wave_primitives():
+    from enterprise_amd.bfs import selftest_wave
+    selftest_wave(0)
+
+
+@pytest.fixture(scope="module")
+def rmat16():
+    g = HostGraph.rmat(16, 16, seed=16)
+    dg = DeviceGraph(g)
+    yield g, dg
+    dg.free()
+
+
+@pytest.mark.parametrize("launch", list(LAUNCHES))
+@pytest.mark.parametrize("name", list(graphs.CLOSED_FORMS))
+def test_closed_forms(oracle, name, launch):
+    g, depth_fn = graphs.CLOSED_FORMS[name]()
+    roots = sorted({0, g.n // 2, g.n - 1})
+    with DeviceGraph(g) as dg:
+        for dname, d in DIRS.items():
+            dg.configure(direction=d, launch=LAUNCHES[launch])
+            for r in roots:
+                _check(oracle, g, dg, r, f"{name}/{dname}/{launch}")
+                got, _ = dg.bfs(r)
+                assert np.array_equal(got, np.asarray(depth_fn(r), dtype=np.int32)), (name, r)
+
+
+@pytest.mark.parametrize("launch", list(LAUNCHES))
+def test_rmat16_config1(oracle, rmat16, launch):
+    g, dg = rmat16
+    root = int(g.pick_roots(1, 1001)[0])
+    for dname, d in DIRS.items():
+        dg.configure(direction=d, launch=LAUNCHES[launch])
+        _check(oracle, g, dg, root, f"rmat16/{dname}/{launch}")
+
+
+@pytest.mark.parametrize("launch", list(LAUNCHES))
+def test_rmat16_all_classes(oracle, rmat16, launch):
+    g, dg = rmat16
+    base = dg.config
+    try:
+        for dname, d in DIRS.items():
+            dg.configure(direction=d, launch=LAUNCHES[launch], **TINY_CLASSES)
+            for r in g.pick_roots(3, 77):
+                _check(oracle, g, dg, int(r), f"rmat16-tiny/{dname}/{launch}")
+    finally:
+        dg.set_config(base)
+
+
+def test_mega_hub_grid_class(oracle):
+    g, _ = graphs.star(70000)
+    with DeviceGraph(g) as dg:
+        for launch in LAUNCHES.values():
+            for d in DIRS.values():
+                dg.configure(direction=d, launch=launch)
+                for r in (0, 1, 69999):
+                    _check(oracle, g, dg, r, f"star70000/{d}/{launch}")
+
+
+@pytest.mark.parametrize("bs", [64, 128, 512, 1024])
+def test_block_sizes(oracle, rmat16, bs):
+    g, dg = rmat16
+    base = dg.config
+    try:
+        for launch in LAUNCHES.values():
+            dg.configure(block_size=bs, launch=launch, direction=L.DIR_AUTO)
+            for r in g.pick_roots(2, 5):
+                _check(oracle, g, dg, int(r), f"rmat16/bs{bs}/{launch}")
+    finally:
+        dg.set_config(base)
+
+
+def test_isolated_root(oracle):
+    g = graphs.edges_graph(10, [(1, 2), (2, 3)])
+    with DeviceGraph(g) as dg:
+        for launch in LAUNCHES.values():
+            dg.configure(launch=launch)
+            got, st = dg.bfs(0)
+            assert got[0] == 0 and (got[1:] == -1).all()
+            assert st["levels"] == 1 and st["reached_edges"] == 0
+
+
+def test_many_roots_schedule_independent(oracle, rmat16):
+    """Forced top-down, bottom-up, reference and auto schedules give the same
+    depths for many roots (the depths do not depend on the schedule)."""
+    g, dg = rmat16
+    roots = g.pick_roots(12, 2024)
+    base = dg.config
+    try:
+        for r in roots:
+            want = oracle.bfs(g.beg_pos, g.adj, int(r))
+            for d in DIRS.values():
+                dg.configure(direction=d, launch=L.LAUNCH_STREAM)
+                got, _ = dg.bfs(int(r))
+                assert np.array_equal(got, want)
+    finally:
+        dg.set_config(base)
