@@ -1,0 +1,205 @@
+"""Benchmark of the MI355X direction-optimizing BFS.
+
+Default workload (one step = one single-source BFS over the resident graph):
+R-MAT scale 22, edgefactor 16 (A/B/C/D = .57/.19/.19/.05, seed 22, relabelled,
+symmetrized, loops and duplicates removed), 64 seeded non-isolated roots
+(seed 1002) run one at a time. Each step is timed on the host around the C-ABI
+call (the reference's window, bfs_gpu_opt.cuh:381-411: from the call to the
+end of the search, depth array left in HBM). The reported value is the
+harmonic mean of per-root GTEPS with the reference's numerator M_c = sum of
+degrees over reached vertices (validate.h:115-123).
+
+Other modes (--mode): grid (4096^2 high-diameter grid, per-level cost of the
+launch forms), batch (512-source multi-source BFS on R-MAT scale 23) and serve
+(continuous-batching serving loop); each prints its own JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "GTEPS (hmean, 64 roots) plain vs batched; HBM GB/s; p99 query ms, 1 MI355X"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def hmean(xs):
+    xs = [x for x in xs if x > 0]
+    return len(xs) / sum(1.0 / x for x in xs) if xs else 0.0
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline_serial(g, roots, budget_s, gpu_depths=None):
+    """Serial queue BFS of the oracle on one host core over a bounded sample
+    of the same roots; also checks the GPU depths of those roots."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ct
+    oracle = oracle_ct.Oracle()
+    rates, checked, mismatches = [], 0, 0
+    t_start = time.perf_counter()
+    for i, r in enumerate(roots):
+        t0 = time.perf_counter()
+        d = oracle.bfs(g.beg_pos, g.adj, int(r))
+        dt = time.perf_counter() - t0
+        edges, _ = oracle.report(g.beg_pos, d)
+        if d.max() > 2:
+            rates.append(edges / dt / 1e9)
+        if gpu_depths is not None and i in gpu_depths:
+            checked += 1
+            mismatches += int(not np.array_equal(d, gpu_depths[i]))
+        if time.perf_counter() - t_start > budget_s:
+            break
+    return {"value": round(hmean(rates), 4), "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": f"serial queue BFS (oracle) on {len(rates)} of the {len(roots)} roots, "
+                      f"R-MAT scale {g.scale_tag}, host {cpu_model()}, "
+                      f"{os.cpu_count()} host cores visible"}, checked, mismatches
+
+
+def load_tuned(dg, name):
+    path = os.path.join(ROOT, "enterprise_amd", "tuned", f"{name}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            cfg = json.load(f)
+        dg.configure(**{k: v for k, v in cfg.get("config", cfg).items() if k != "reserved"})
+        return path
+    return None
+
+
+def bench_plain(args):
+    from enterprise_amd import DeviceGraph, HostGraph
+    from enterprise_amd import _lib as L
+
+    scale = args.scale
+    t0 = time.perf_counter()
+    g = HostGraph.rmat(scale, 16, seed=scale)
+    g.scale_tag = scale
+    log(f"[bench] R-MAT scale {scale}: n={g.n} m={g.m} generated in {time.perf_counter() - t0:.1f}s")
+    roots = g.pick_roots(64, 1002)
+    dg = DeviceGraph(g, device=0)
+    tuned = load_tuned(dg, f"rmat{scale}")
+    if args.launch is not None:
+        dg.configure(launch={"persistent": L.LAUNCH_PERSISTENT, "levels": L.LAUNCH_LEVELS,
+                             "graph": L.LAUNCH_GRAPH, "stream": L.LAUNCH_STREAM}[args.launch])
+    log(f"[bench] config {dg.config.as_dict()} (tuned file: {tuned})")
+    for i in range(args.warmup):
+        dg.bfs_device(int(roots[i % 64]))
+    log("[bench] warmup done")
+    per = []
+    gpu_depths = {}
+    dg_sync = L.lib()
+    t_all0 = time.perf_counter()
+    for k in range(args.steps):
+        r = int(roots[k % 64])
+        t0 = time.perf_counter()
+        st = dg.bfs_device(r)
+        dt = time.perf_counter() - t0
+        per.append((r, dt, st))
+    t_all = time.perf_counter() - t_all0
+    del dg_sync
+    log(f"[bench] timed {args.steps} BFS in {t_all:.3f}s")
+    # Depths of a few timed roots for the CPU check (outside the timed region).
+    for i in range(min(args.check_roots, 64)):
+        d, _ = dg.bfs(int(roots[i]))
+        gpu_depths[i] = d
+    used = [(r, dt, st) for (r, dt, st) in per if st["levels"] - 1 > 2]
+    gteps = [st["reached_edges"] / dt / 1e9 for (r, dt, st) in used]
+    gteps_dev = [st["reached_edges"] / (st["device_ms"] * 1e-3) / 1e9 for (r, dt, st) in used]
+    alg_bytes = [st["alg_bytes"] for (_, _, st) in used]
+    dev_ms = [st["device_ms"] for (_, _, st) in used]
+    host_ms = [dt * 1e3 for (_, dt, _) in used]
+    achieved = sum(alg_bytes) / (sum(dev_ms) * 1e-3) / 1e9  # GB/s over device time
+    achieved_host = sum(alg_bytes) / (sum(host_ms) * 1e-3) / 1e9
+    value = hmean(gteps)
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        cpu, checked, mism = cpu_baseline_serial(g, roots, args.cpu_budget, gpu_depths)
+        parity = {"roots_checked": checked, "mismatches": mism}
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(float(np.mean(host_ms)), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"plain single-source DO-BFS, R-MAT scale {scale} ef 16 "
+                               f"(a=.57 b=.19 c=.19, seed {scale}), 64 roots (seed 1002)",
+                   "n": g.n, "m_directed": g.m, "roots": 64,
+                   "launch": dg.config.launch, "direction": dg.config.direction},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": None,
+                     "kernel": "whole search (init + level kernels) per BFS, HIP events on "
+                               "the handle's stream",
+                     "alg_bytes_per_bfs_mean": int(np.mean(alg_bytes))},
+        "cpu_baseline": cpu,
+        "extra": {
+            "gteps_hmean_device_time": round(hmean(gteps_dev), 3),
+            "gteps_amean": round(float(np.mean(gteps)), 3),
+            "gteps_graph500_undirected_hmean": round(value / 2, 3),
+            "alg_GBps_host_window": round(achieved_host, 1),
+            "device_ms_mean": round(float(np.mean(dev_ms)), 4),
+            "host_ms_p50": round(float(np.percentile(host_ms, 50)), 4),
+            "levels_mean": round(float(np.mean([st["levels"] for (_, _, st) in used])), 2),
+            "roots_used": len(used),
+            "parity": parity,
+        },
+    }
+    dg.free()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--mode", default="plain", choices=["plain", "grid", "batch", "serve"])
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--launch", default=None, choices=["persistent", "levels", "graph", "stream"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--check-roots", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(batched and serving work shard by replicating the graph, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.mode == "plain":
+        out = bench_plain(args)
+    else:
+        import bench_modes
+        out = bench_modes.run(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -50,7 +50,8 @@ class Stats(C.Structure):
                 ("bu_levels", C.c_int32), ("phases", C.c_int32),
                 ("reached_vertices", C.c_int64), ("reached_edges", C.c_int64),
                 ("alg_bytes", C.c_int64), ("device_ms", C.c_double),
-                ("host_ms", C.c_double), ("bu_level_mask", C.c_uint64)]
+                ("host_ms", C.c_double), ("bu_level_mask", C.c_uint64),
+                ("grid_blocks", C.c_int32), ("block_size", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

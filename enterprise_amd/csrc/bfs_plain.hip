@@ -611,6 +611,7 @@ struct ent_graph {
   Args graph_args{};
   int graph_bs = 0, graph_grid = 0, graph_k = 0;
   size_t dev_bytes = 0;
+  int last_grid = 0;
   std::mutex mu;
 };
 
@@ -635,26 +636,29 @@ int dev_alloc(ent_graph *g, void **p, size_t bytes) {
   return ENT_OK;
 }
 
+template <class T>
+inline void ignore(T) {}
+
 void release(ent_graph *g) {
   if (!g) return;
-  hipSetDevice(g->device);
-  if (g->gexec) hipGraphExecDestroy(g->gexec);
-  if (g->graph) hipGraphDestroy(g->graph);
-  hipFree(g->off);
-  hipFree(g->adj);
-  hipFree(g->nzmask);
-  hipFree(g->depth);
-  hipFree(g->visited);
+  ignore(hipSetDevice(g->device));
+  if (g->gexec) ignore(hipGraphExecDestroy(g->gexec));
+  if (g->graph) ignore(hipGraphDestroy(g->graph));
+  ignore(hipFree(g->off));
+  ignore(hipFree(g->adj));
+  ignore(hipFree(g->nzmask));
+  ignore(hipFree(g->depth));
+  ignore(hipFree(g->visited));
   for (int p = 0; p < 2; ++p) {
-    hipFree(g->front[p]);
-    for (int c = 0; c < 3; ++c) hipFree(g->q[p][c]);
-    hipFree(g->qg[p]);
+    ignore(hipFree(g->front[p]));
+    for (int c = 0; c < 3; ++c) ignore(hipFree(g->q[p][c]));
+    ignore(hipFree(g->qg[p]));
   }
-  hipFree(g->ctl);
-  if (g->h_ctl) hipHostFree(g->h_ctl);
-  if (g->ev0) hipEventDestroy(g->ev0);
-  if (g->ev1) hipEventDestroy(g->ev1);
-  if (g->stream) hipStreamDestroy(g->stream);
+  ignore(hipFree(g->ctl));
+  if (g->h_ctl) ignore(hipHostFree(g->h_ctl));
+  if (g->ev0) ignore(hipEventDestroy(g->ev0));
+  if (g->ev1) ignore(hipEventDestroy(g->ev1));
+  if (g->stream) ignore(hipStreamDestroy(g->stream));
   delete g;
 }
 
@@ -745,11 +749,41 @@ hipError_t launch(const void *fn, int grid, int bs, void **args, hipStream_t s) 
 hipError_t queue_levels(ent_graph *g, const KernelSet &ks, Args *a, int k) {
   void *args[] = {a};
   const int grid = level_grid(g);
+  g->last_grid = grid;
   for (int i = 0; i < k; ++i) {
     hipError_t e = launch(ks.level, grid, g->cfg.block_size, args, g->stream);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+size_t smem_bytes(int bs) {
+  switch (bs) {
+    case 64: return sizeof(Smem<64>);
+    case 128: return sizeof(Smem<128>);
+    case 512: return sizeof(Smem<512>);
+    case 1024: return sizeof(Smem<1024>);
+    default: return sizeof(Smem<256>);
+  }
+}
+
+// Blocks per CU for the cooperative kernel: the occupancy query, capped by
+// the LDS budget (160 KiB per CU, each block's LDS rounded up to a 2 KiB
+// allocation granule), 32 wavefronts per CU and 16 workgroups per CU, and by
+// blocks_per_cu when set. Every block must be resident at once.
+int persistent_blocks_per_cu(ent_graph *g, const KernelSet &ks) {
+  const int bs = g->cfg.block_size;
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ks.persist, bs, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  const size_t lds_block = (smem_bytes(bs) + 2047) / 2048 * 2048;
+  const int lds_limit = int((160 * 1024) / lds_block);
+  const int wave_limit = 32 / (bs / kWave);
+  int per_cu = std::min(std::min(occ, lds_limit), std::min(wave_limit, 16));
+  if (g->cfg.blocks_per_cu > 0) per_cu = std::min(per_cu, g->cfg.blocks_per_cu);
+  return per_cu;
 }
 
 int run_search(ent_graph *g, int root, int *depth_dev) {
@@ -766,12 +800,11 @@ int run_search(ent_graph *g, int root, int *depth_dev) {
   const int64_t max_phases = 2 * g->n + 16;
   switch (cfg.launch) {
     case ENT_LAUNCH_PERSISTENT: {
-      int per_cu = 0;
-      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ks.persist, bs, 0));
+      const int per_cu = persistent_blocks_per_cu(g, ks);
       if (per_cu < 1) return set_error(ENT_EDEVICE, "persistent kernel cannot be resident");
-      if (cfg.blocks_per_cu > 0) per_cu = std::min(per_cu, cfg.blocks_per_cu);
       void *args[] = {&a};
-      HIP_TRY(hipLaunchCooperativeKernel(ks.persist, dim3(g->cus * per_cu), dim3(bs), args, 0, g->stream));
+      g->last_grid = g->cus * per_cu;
+      HIP_TRY(hipLaunchCooperativeKernel(ks.persist, dim3(g->last_grid), dim3(bs), args, 0, g->stream));
       break;
     }
     case ENT_LAUNCH_LEVELS:
@@ -791,8 +824,8 @@ int run_search(ent_graph *g, int root, int *depth_dev) {
       const int grid = level_grid(g);
       if (!g->gexec || g->graph_bs != bs || g->graph_grid != grid || g->graph_k != k ||
           std::memcmp(&g->graph_args, &a, sizeof(Args)) != 0) {
-        if (g->gexec) hipGraphExecDestroy(g->gexec);
-        if (g->graph) hipGraphDestroy(g->graph);
+        if (g->gexec) ignore(hipGraphExecDestroy(g->gexec));
+        if (g->graph) ignore(hipGraphDestroy(g->graph));
         g->gexec = nullptr;
         g->graph = nullptr;
         HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
@@ -938,8 +971,8 @@ int ent_graph_free(ent_graph *g) {
   if (!g) return ENT_OK;
   {
     std::lock_guard<std::mutex> lk(g->mu);
-    hipSetDevice(g->device);
-    hipStreamSynchronize(g->stream);
+    ignore(hipSetDevice(g->device));
+    ignore(hipStreamSynchronize(g->stream));
   }
   release(g);
   return ENT_OK;
@@ -966,7 +999,7 @@ int ent_set_config(ent_graph *g, const ent_bfs_config *cfg) {
     HIP_TRY(hipSetDevice(g->device));
     HIP_TRY(hipStreamSynchronize(g->stream));
     for (int p = 0; p < 2; ++p) {
-      hipFree(g->qg[p]);
+      ignore(hipFree(g->qg[p]));
       g->qg[p] = nullptr;
       rc = dev_alloc(g, reinterpret_cast<void **>(&g->qg[p]), sizeof(long long) * size_t(need));
       if (rc) return rc;
@@ -997,7 +1030,7 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
   HIP_TRY(hipEventRecord(g->ev0, g->stream));
   int rc = run_search(g, root, depth);
   if (rc) {
-    hipStreamSynchronize(g->stream);
+    ignore(hipStreamSynchronize(g->stream));
     return rc;
   }
   HIP_TRY(hipEventRecord(g->ev1, g->stream));
@@ -1022,6 +1055,8 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
     if (hipEventElapsedTime(&ms, g->ev0, g->ev1) == hipSuccess) st->device_ms = ms;
     st->host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     st->bu_level_mask = c.bu_mask;
+    st->grid_blocks = g->last_grid;
+    st->block_size = g->cfg.block_size;
   }
   return ENT_OK;
 }
@@ -1085,7 +1120,7 @@ int ent_selftest_wave(int device) {
   std::vector<int> h(256);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpy(h.data(), d, 256 * sizeof(int), hipMemcpyDeviceToHost);
-  hipFree(d);
+  ignore(hipFree(d));
   if (e != hipSuccess) return set_error(ENT_EDEVICE, "selftest: %s", hipGetErrorString(e));
   int run = 0, odd_rank = 0;
   for (int lane = 0; lane < 64; ++lane) {

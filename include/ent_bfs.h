@@ -84,6 +84,8 @@ typedef struct ent_stats {
   double device_ms;         /* device time of the search (HIP events) */
   double host_ms;           /* wall time of the call */
   uint64_t bu_level_mask;   /* bit L set: level L ran bottom-up (L < 64) */
+  int32_t grid_blocks;      /* blocks per level launch / persistent grid */
+  int32_t block_size;       /* threads per block */
 } ent_stats;
 
 /* Fills cfg with the library defaults. */

@@ -1,12 +1,13 @@
-"""ctypes access to the CPU oracle (oracle/libent_oracle.so). Test
-infrastructure: the checker, never the thing measured."""
+"""ctypes access to the CPU oracle (oracle/libent_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the CPU-baseline
+leg of bench.py as the checker, never as the measured or shipped path."""
 import ctypes as C
 import os
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "libent_oracle.so")
+ORACLE_SO = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libent_oracle.so")
 
 i32p = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
