@@ -3,30 +3,44 @@
 //
 // Reference behaviour restated (iHeartGraph/Enterprise):
 //  - level loop, direction state and termination: bfs_tdbu_clfy_sort,
-//    bfs_gpu_opt.cuh:40-288; termination when a level finds nothing
+//    bfs_gpu_opt.cuh:40-288; the search ends when a level finds nothing
 //    (:193-202);
-//  - top-down expansion with degree classes: THD/WAP/CTA_expand_sort,
-//    expander.cuh:77-423 (warp32 classes, thresholds 32 / 1024 from
-//    bfs_gpu_opt.cuh:323-324) -> here thread / wavefront / block / grid
-//    classes sized for 64-lane wavefronts;
+//  - top-down expansion by degree class: THD/WAP/CTA_expand_sort,
+//    expander.cuh:77-423 (warp32 classes split at 32 and 1024 edges,
+//    bfs_gpu_opt.cuh:323-324); unvisited neighbours get LEVEL through plain,
+//    idempotent writes (expander.cuh:162-163; the reference has no atomics);
 //  - bottom-up expansion with early exit at the first frontier neighbour:
 //    THD/WAP/CTA_bu_expand_sort, expander.cuh:434-819;
-//  - status-array inspection and classification into class queues:
-//    inspector.cuh:38-432 plus the bin-offset scans of scan.cuh:485-685 ->
-//    here ballot/DPP ranks and LDS staging, no per-thread bins, no reaper;
+//  - status-array inspection (inspector.cuh:38-432) and the bin-offset scans
+//    of scan.cuh:485-685;
 //  - direction switch: inspector.cuh:486-499 (0.3 * m, one way), offered as
 //    ENT_DIR_REFERENCE beside a Beamer alpha/beta rule.
 //
-// Device state (all in HBM, sized at upload):
-//   off int64[n+1], adj int32[m], nzmask u64[nwords] (bit = degree > 0),
-//   depth int32[n], visited u64[nwords], front[2] u64[nwords],
-//   class queues q[2][3] int32[n] (thread / wave / block class) and grid
-//   pieces qg[2] int64 ((v << 32) | piece index), Control (level state and
-//   three rotating per-level counter slots).
+// MI355X design:
+//  - The frontier of every level is a bitmap (one u64 word per 64 vertices).
+//    Thread- and wavefront-class frontier vertices are found by sweeping that
+//    bitmap with ballot/popcount/DPP compaction; block- and grid-class
+//    vertices (degree >= t_block) come from static per-graph hub lists that
+//    are filtered through the bitmap, so no per-level queue is built.
+//  - Top-down writes one byte mark per discovered vertex with plain stores
+//    (no atomics, no returned values on the critical path); an inspection
+//    pass turns marks into the next frontier bitmap, the depth writes and the
+//    frontier counts, one wavefront per bitmap word (coalesced).
+//  - Bottom-up owns whole bitmap words per wavefront, so it updates visited,
+//    next frontier and depth without atomics.
+//  - Counters and the end-of-phase arrival are striped over 64 cache lines so
+//    at most ~32 blocks contend for one address.
+//
+// Device state (HBM, sized at upload):
+//   off int64[n+1], adj int32[m], nzmask u64[W] (degree > 0), W = ceil(n/64)
+//   hub_block int32[] (t_block <= degree < t_grid), piece_v/piece_k int32[]
+//   (degree >= t_grid, split into `piece`-edge pieces)
+//   depth int32[n], visited u64[W], front[2] u64[W], mark u8[n], Control.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -40,13 +54,14 @@
 namespace ent {
 
 enum Phase : int { PH_TD = 0, PH_BU = 1, PH_INSPECT = 2 };
-enum : int { ERR_NONE = 0, ERR_OVERFLOW = 1, ERR_LEVELS = 2, ERR_TIMEOUT = 4 };
+enum : int { ERR_NONE = 0, ERR_LEVELS = 2, ERR_TIMEOUT = 4 };
 
-struct LevelSlot {
-  unsigned int qsize[4];  // class queue sizes of the frontier (class 3: pieces)
-  unsigned int nf;        // frontier vertices
-  unsigned int pad;
-  unsigned long long mf;  // sum of frontier degrees
+constexpr int kStripes = 64;     // counter / arrival stripes
+constexpr int kStripeU64 = 16;   // 128 B between stripes
+
+struct Counters {
+  // stripe s: nf at [s * kStripeU64], mf at [s * kStripeU64 + 1]
+  unsigned long long v[kStripes * kStripeU64];
 };
 
 struct Control {
@@ -54,191 +69,210 @@ struct Control {
   int phase;
   int done;
   int error;
-  int bitmap_valid;  // frontier of `level` is stored in front[level & 1]
   int levels;
   int td_levels;
   int bu_levels;
   int phases;
-  unsigned int arrive;
-  unsigned int gen;
-  int pad;
   unsigned long long total_nf;
   unsigned long long total_mf;
   unsigned long long prev_nf;
   unsigned long long bu_mask;
-  LevelSlot slot[3];
+  unsigned int arrive_all;
+  unsigned int gen;
+  unsigned int pad[26];
+  unsigned int arrive[kStripes * 32];  // one 128 B line per stripe
+  Counters cnt[3];                     // frontier counts by level % 3
 };
 
 struct Args {
   const long long *off;
   const int *adj;
   const unsigned long long *nzmask;
+  const int *hub_block;   // static block-class list
+  const int *piece_v;     // static grid-class pieces: vertex
+  const int *piece_k;     // piece index within the vertex
   int *depth;
   unsigned long long *visited;
   unsigned long long *front[2];
-  int *q[2][3];
-  long long *qg[2];
+  unsigned char *mark;
   Control *ctl;
   long long m;
   long long n_active;
-  long long cap3;
   int n;
   int nwords;
-  int t_wave, t_block, t_grid, piece;
+  int n_hub_block;
+  int n_pieces;
+  int t_wave, t_block, piece;
   int mode, alpha, beta;
   int bu_probe;
-  int bu_chunk_words;
+  int chunk_words;
 };
 
 template <int BS>
 struct Smem {
   static constexpr int NW = BS / kWave;
-  static constexpr int S0 = 2048, S1 = 512, S2 = 128;
-  union {
-    struct {
-      int buf0[S0];
-      int buf1[S1];
-      int buf2[S2];
-    } td;
-    struct {
-      unsigned long long cand[NW][kWave];
-      unsigned long long found[NW][kWave];
-      int excl[NW][kWave];
-    } bu;
-  } u;
-  int cnt[3];
-  unsigned int base[3];
+  unsigned long long cand[NW][kWave];
+  unsigned long long found[NW][kWave];
+  int excl[NW][kWave];
   unsigned long long nf, mf;
-  int L, ph, done, bm;
+  int L, ph, done;
 };
 
-__device__ __forceinline__ bool in_front(const Args &a, const unsigned long long *fr, bool bm, int u,
-                                         int L) {
-  return bm ? ((fr[u >> 6] >> (u & 63)) & 1ull) != 0 : a.depth[u] == L;
+__device__ __forceinline__ bool bit_of(const unsigned long long *bm, int v) {
+  return ((bm[v >> 6] >> (v & 63)) & 1ull) != 0;
 }
 
-// Appends v (degree e - s) to the class queues of parity `par`, counting it
-// in `slot`. Classes 0..2 are staged in LDS and flushed per block; grid-class
-// vertices are split into pieces written straight to the global piece queue.
+// Adds a block's frontier counts to its stripe.
 template <int BS>
-__device__ __forceinline__ void push_vertex(const Args &a, Smem<BS> &sm, int v, long long s, long long e,
-                                            int par, LevelSlot *slot) {
-  const long long d = e - s;
-  atomicAdd(&sm.nf, 1ull);
-  atomicAdd(&sm.mf, (unsigned long long)d);
-  if (d >= a.t_grid) {
-    const unsigned np = unsigned((d + a.piece - 1) / a.piece);
-    const unsigned base = atomicAdd(&slot->qsize[3], np);
-    // Every reserved index below cap3 is written, so a reader clamped to cap3
-    // never sees an unwritten entry.
-    for (unsigned k = 0; k < np && (long long)base + k < a.cap3; ++k)
-      a.qg[par][base + k] = ((long long)v << 32) | k;
-    if ((long long)base + np > a.cap3) atomicOr(&a.ctl->error, ERR_OVERFLOW);
-    return;
-  }
-  const int c = d >= a.t_block ? 2 : (d >= a.t_wave ? 1 : 0);
-  const int i = atomicAdd(&sm.cnt[c], 1);
-  int *buf = c == 0 ? sm.u.td.buf0 : (c == 1 ? sm.u.td.buf1 : sm.u.td.buf2);
-  const int cap = c == 0 ? Smem<BS>::S0 : (c == 1 ? Smem<BS>::S1 : Smem<BS>::S2);
-  if (i < cap) {
-    buf[i] = v;
-  } else {
-    const unsigned g = atomicAdd(&slot->qsize[c], 1u);
-    if (g < unsigned(a.n)) a.q[par][c][g] = v;
-    else atomicOr(&a.ctl->error, ERR_OVERFLOW);
+__device__ __forceinline__ void add_counts(const Args &a, Smem<BS> &sm, int slot) {
+  __syncthreads();
+  if (threadIdx.x == 0 && sm.nf) {
+    unsigned long long *c = a.ctl->cnt[slot].v + (blockIdx.x % kStripes) * kStripeU64;
+    atomicAdd(c, sm.nf);
+    atomicAdd(c + 1, sm.mf);
   }
 }
 
-// Moves the block's LDS-staged queue entries to global memory with one
-// reservation per class, and adds the block's vertex / degree counts.
+// Sums a per-lane count over the wavefront and adds it to the block total.
 template <int BS>
-__device__ void flush_push(const Args &a, Smem<BS> &sm, int par, LevelSlot *slot, bool count) {
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    const int c = threadIdx.x;
-    const int cap = c == 0 ? Smem<BS>::S0 : (c == 1 ? Smem<BS>::S1 : Smem<BS>::S2);
-    const int k = min(sm.cnt[c], cap);
-    sm.base[c] = k ? atomicAdd(&slot->qsize[c], unsigned(k)) : 0u;
-  } else if (threadIdx.x == 3 && count && sm.nf) {
-    atomicAdd(&slot->nf, unsigned(sm.nf));
-    atomicAdd(&slot->mf, sm.mf);
-  }
-  __syncthreads();
+__device__ __forceinline__ void wave_add_counts(Smem<BS> &sm, unsigned long long nf, unsigned long long mf) {
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const int cap = c == 0 ? Smem<BS>::S0 : (c == 1 ? Smem<BS>::S1 : Smem<BS>::S2);
-    const int k = min(sm.cnt[c], cap);
-    const int *buf = c == 0 ? sm.u.td.buf0 : (c == 1 ? sm.u.td.buf1 : sm.u.td.buf2);
-    const unsigned base = sm.base[c];
-    int *dst = a.q[par][c];
-    for (int i = threadIdx.x; i < k; i += BS) {
-      if (base + unsigned(i) < unsigned(a.n)) dst[base + i] = buf[i];
-    }
-    if (k > 0 && threadIdx.x == 0 && base + unsigned(k) > unsigned(a.n)) atomicOr(&a.ctl->error, ERR_OVERFLOW);
+  for (int o = 32; o > 0; o >>= 1) {
+    nf += __shfl_xor(nf, o);
+    mf += __shfl_xor(mf, o);
   }
-  __syncthreads();
+  if (lane_id() == 0 && nf) {
+    atomicAdd(&sm.nf, nf);
+    atomicAdd(&sm.mf, mf);
+  }
 }
 
-template <int BS>
-__device__ __forceinline__ void td_visit(const Args &a, Smem<BS> &sm, int u, int L, int npar,
-                                         LevelSlot *out) {
-  const unsigned long long bit = 1ull << (u & 63);
-  unsigned long long *w = a.visited + (u >> 6);
-  if (*w & bit) return;  // may be stale; the atomic below decides
-  const unsigned long long old = atomicOr(w, bit);
-  if (old & bit) return;
-  a.depth[u] = L + 1;
-  push_vertex(a, sm, u, a.off[u], a.off[u + 1], npar, out);
+// Wave-level compaction of the set bits of up to 64 words (one per lane,
+// `word` = 0 for lanes without one). Stores the per-lane words and exclusive
+// bit counts in LDS and returns the total number of set bits.
+__device__ __forceinline__ int wave_compact_setup(unsigned long long word, unsigned long long *wc,
+                                                  int *we) {
+  const int lane = lane_id();
+  const int cnt = __popcll(word);
+  const int incl = wave_incl_scan(cnt);
+  wc[lane] = word;
+  we[lane] = incl - cnt;
+  wave_sync();
+  return wave_read(incl, 63);
 }
 
-// Top-down level L: expand the class queues of parity L & 1 (counts in slot
-// L % 3) into those of parity (L + 1) & 1 (slot (L + 1) % 3).
+// Maps compacted index k (< total) to (lane j owning the word, bit position).
+__device__ __forceinline__ void wave_compact_get(const unsigned long long *wc, const int *we, int k,
+                                                 int &j, int &bp) {
+  int lo = 0, hi = kWave - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (we[mid] <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  j = lo;
+  bp = select_bit(wc[j], k - we[j]);
+}
+
+// XCD-aware wavefront numbering: blocks go round-robin over the 8 XCDs, so
+// consecutive chunks are given to blocks of one XCD.
+__device__ __forceinline__ long long xcd_wave_id() {
+  long long eb = blockIdx.x;
+  if ((gridDim.x & 7) == 0) eb = (long long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  return eb * (blockDim.x / kWave) + threadIdx.x / kWave;
+}
+
+__device__ __forceinline__ void td_mark(const Args &a, int u) {
+  if (!bit_of(a.visited, u)) a.mark[u] = 1;
+}
+
+// Top-down level L. Grid-class pieces and block-class hubs come from the
+// static lists (skipped unless in frontier L); thread- and wavefront-class
+// frontier vertices come from a compacting sweep of front[L & 1]. Every
+// unvisited neighbour gets a byte mark; nothing is counted here.
 template <int BS>
 __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
-  LevelSlot *in = &a.ctl->slot[L % 3];
-  LevelSlot *out = &a.ctl->slot[(L + 1) % 3];
-  const int par = L & 1, npar = par ^ 1;
-  const unsigned n0 = min(ld_agent(&in->qsize[0]), unsigned(a.n));
-  const unsigned n1 = min(ld_agent(&in->qsize[1]), unsigned(a.n));
-  const unsigned n2 = min(ld_agent(&in->qsize[2]), unsigned(a.n));
-  const unsigned n3 = unsigned(min((long long)ld_agent(&in->qsize[3]), a.cap3));
-  const int lane = lane_id();
-  // Grid class: one block per piece of a huge adjacency list.
-  for (unsigned i = blockIdx.x; i < n3; i += gridDim.x) {
-    const long long x = a.qg[par][i];
-    const int v = int(x >> 32);
-    const long long k = x & 0xffffffffll;
-    const long long s = a.off[v] + k * a.piece;
+  const unsigned long long *fr = a.front[L & 1];
+  for (int p = blockIdx.x; p < a.n_pieces; p += gridDim.x) {
+    const int v = a.piece_v[p];
+    if (!bit_of(fr, v)) continue;
+    const long long s = a.off[v] + (long long)a.piece_k[p] * a.piece;
     const long long e = min(a.off[v + 1], s + a.piece);
-    for (long long p = s + threadIdx.x; p < e; p += BS) td_visit(a, sm, a.adj[p], L, npar, out);
+    for (long long i = s + threadIdx.x; i < e; i += BS) td_mark(a, a.adj[i]);
   }
-  // Block class.
-  for (unsigned i = blockIdx.x; i < n2; i += gridDim.x) {
-    const int v = a.q[par][2][i];
+  for (int h = blockIdx.x; h < a.n_hub_block; h += gridDim.x) {
+    const int v = a.hub_block[h];
+    if (!bit_of(fr, v)) continue;
     const long long s = a.off[v], e = a.off[v + 1];
-    for (long long p = s + threadIdx.x; p < e; p += BS) td_visit(a, sm, a.adj[p], L, npar, out);
+    for (long long i = s + threadIdx.x; i < e; i += BS) td_mark(a, a.adj[i]);
   }
-  // Wavefront class: 64 lanes stride one adjacency list (coalesced).
-  {
-    const unsigned gw = blockIdx.x * (BS / kWave) + threadIdx.x / kWave;
-    const unsigned nw = gridDim.x * (BS / kWave);
-    for (unsigned i = gw; i < n1; i += nw) {
-      const int v = a.q[par][1][i];
-      const long long s = a.off[v], e = a.off[v + 1];
-      for (long long p = s + lane; p < e; p += kWave) td_visit(a, sm, a.adj[p], L, npar, out);
+  const int wid = threadIdx.x / kWave, lane = lane_id();
+  unsigned long long *wc = sm.cand[wid];
+  int *we = sm.excl[wid];
+  const int CW = a.chunk_words;
+  const long long nchunks = ((long long)a.nwords + CW - 1) / CW;
+  const long long nwv = (long long)gridDim.x * (BS / kWave);
+  for (long long ch = xcd_wave_id(); ch < nchunks; ch += nwv) {
+    const long long w = ch * CW + lane;
+    const unsigned long long word = (lane < CW && w < a.nwords) ? fr[w] : 0ull;
+    if (__ballot(word != 0) == 0) continue;
+    const int total = wave_compact_setup(word, wc, we);
+    for (int b0 = 0; b0 < total; b0 += kWave) {
+      const int k = b0 + lane;
+      long long s = 0, e = 0;
+      if (k < total) {
+        int j, bp;
+        wave_compact_get(wc, we, k, j, bp);
+        const int v = int((ch * CW + j) * 64 + bp);
+        s = a.off[v];
+        e = a.off[v + 1];
+        if (e - s < a.t_wave) {
+          for (long long i = s; i < e; ++i) td_mark(a, a.adj[i]);
+        }
+      }
+      const long long d = e - s;
+      unsigned long long wm = __ballot(k < total && d >= a.t_wave && d < a.t_block);
+      while (wm) {
+        const int src = __ffsll((long long)wm) - 1;
+        wm &= wm - 1;
+        const long long ss = __shfl(s, src), ee = __shfl(e, src);
+        for (long long i = ss + lane; i < ee; i += kWave) td_mark(a, a.adj[i]);
+      }
     }
+    wave_sync();
   }
-  // Thread class: one lane per short adjacency list.
-  {
-    const unsigned gt = blockIdx.x * BS + threadIdx.x;
-    const unsigned nt = gridDim.x * BS;
-    for (unsigned i = gt; i < n0; i += nt) {
-      const int v = a.q[par][0][i];
-      const long long s = a.off[v], e = a.off[v + 1];
-      for (long long p = s; p < e; ++p) td_visit(a, sm, a.adj[p], L, npar, out);
+}
+
+// Inspection after a top-down level L: one wavefront per bitmap word (lane b
+// is vertex 64w + b). New frontier = marked and unvisited. Writes visited,
+// front[(L + 1) & 1] (every word), depth L + 1, clears the marks and counts
+// the new frontier into cnt[(L + 1) % 3].
+template <int BS>
+__device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L) {
+  unsigned long long *fn = a.front[(L + 1) & 1];
+  const int lane = lane_id();
+  const long long nwv = (long long)gridDim.x * (BS / kWave);
+  unsigned long long nf = 0, mf = 0;
+  for (long long w = xcd_wave_id(); w < a.nwords; w += nwv) {
+    const unsigned long long vis = a.visited[w];
+    unsigned long long nw = 0;
+    if (~vis) {
+      const long long v = w * 64 + lane;
+      const bool unvisited = ((vis >> lane) & 1ull) == 0;  // implies v < n
+      const bool mk = unvisited && a.mark[v] != 0;
+      nw = __ballot(mk);
+      if (mk) {
+        a.mark[v] = 0;
+        a.depth[v] = L + 1;
+        nf += 1;
+        mf += (unsigned long long)(a.off[v + 1] - a.off[v]);
+      }
+      if (lane == 0 && nw) a.visited[w] = vis | nw;
     }
+    if (lane == 0) fn[w] = nw;
   }
-  flush_push(a, sm, npar, out, true);
+  wave_add_counts(sm, nf, mf);
+  add_counts(a, sm, (L + 1) % 3);
 }
 
 // Bottom-up level L: every unvisited vertex with neighbours looks for a
@@ -247,38 +281,26 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
 // vertices (popcount + DPP scan), gives each lane one candidate for up to
 // bu_probe neighbours, hands longer unresolved lists to the whole wavefront
 // (ballot early exit), then writes its words of the next frontier and of the
-// visited bitmap without atomics.
+// visited bitmap without atomics, the depths, and the counts.
 template <int BS>
-__device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, bool use_bitmap) {
-  LevelSlot *out = &a.ctl->slot[(L + 1) % 3];
+__device__ void bu_phase(const Args &a, Smem<BS> &sm, int L) {
   const unsigned long long *fr = a.front[L & 1];
   unsigned long long *fn = a.front[(L + 1) & 1];
   const int wid = threadIdx.x / kWave, lane = lane_id();
-  unsigned long long *wc = sm.u.bu.cand[wid];
-  unsigned long long *wf = sm.u.bu.found[wid];
-  int *we = sm.u.bu.excl[wid];
-  const int CW = a.bu_chunk_words;
+  unsigned long long *wc = sm.cand[wid];
+  unsigned long long *wf = sm.found[wid];
+  int *we = sm.excl[wid];
+  const int CW = a.chunk_words;
   const long long nchunks = ((long long)a.nwords + CW - 1) / CW;
-  // XCD-aware numbering: blocks are dispatched round-robin over the 8 XCDs,
-  // so give each XCD a contiguous range of chunks.
-  long long eb = blockIdx.x;
-  if ((gridDim.x & 7) == 0) eb = (long long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-  const long long gw = eb * (BS / kWave) + wid;
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long nf = 0, mf = 0;
-  for (long long ch = gw; ch < nchunks; ch += nwv) {
+  for (long long ch = xcd_wave_id(); ch < nchunks; ch += nwv) {
     const long long w0 = ch * CW;
     const long long w = w0 + lane;
     const bool own = lane < CW && w < a.nwords;
     const unsigned long long vis = own ? a.visited[w] : ~0ull;
-    const unsigned long long cand = ~vis;
-    const int cnt = __popcll(cand);
-    const int incl = wave_incl_scan(cnt);
-    const int total = wave_read(incl, 63);
-    wc[lane] = cand;
-    we[lane] = incl - cnt;
     wf[lane] = 0;
-    wave_sync();
+    const int total = wave_compact_setup(~vis, wc, we);
     for (int b0 = 0; b0 < total; b0 += kWave) {
       const int k = b0 + lane;
       int v = 0, j = 0;
@@ -286,21 +308,15 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, bool use_bitmap) {
       unsigned long long bitm = 0;
       bool found = false;
       if (k < total) {
-        int lo = 0, hi = kWave - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (we[mid] <= k) lo = mid;
-          else hi = mid - 1;
-        }
-        j = lo;
-        const int bp = select_bit(wc[j], k - we[j]);
+        int bp;
+        wave_compact_get(wc, we, k, j, bp);
         v = int((w0 + j) * 64 + bp);
         bitm = 1ull << bp;
         s = a.off[v];
         e = a.off[v + 1];
         const long long pe = min(e, s + a.bu_probe);
         for (p = s; p < pe; ++p) {
-          if (in_front(a, fr, use_bitmap, a.adj[p], L)) {
+          if (bit_of(fr, a.adj[p])) {
             found = true;
             break;
           }
@@ -325,7 +341,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, bool use_bitmap) {
         bool hit_any = false;
         for (long long q0 = pp; q0 < ee; q0 += kWave) {
           const long long q = q0 + lane;
-          const bool hit = q < ee && in_front(a, fr, use_bitmap, a.adj[q], L);
+          const bool hit = q < ee && bit_of(fr, a.adj[q]);
           if (__ballot(hit)) {
             hit_any = true;
             break;
@@ -347,52 +363,30 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, bool use_bitmap) {
     }
     wave_sync();
   }
-  if (nf) {
-    atomicAdd(&sm.nf, nf);
-    atomicAdd(&sm.mf, mf);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && sm.nf) {
-    atomicAdd(&out->nf, unsigned(sm.nf));
-    atomicAdd(&out->mf, sm.mf);
-  }
-  __syncthreads();
+  wave_add_counts(sm, nf, mf);
+  add_counts(a, sm, (L + 1) % 3);
 }
 
-// Inspection after bottom-up: turns the frontier bitmap of level L into the
-// class queues of parity L & 1 (slot L % 3) for a top-down expansion.
-template <int BS>
-__device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L) {
-  LevelSlot *slot = &a.ctl->slot[L % 3];
-  const unsigned long long *fr = a.front[L & 1];
-  const int par = L & 1;
-  const long long gt = (long long)blockIdx.x * BS + threadIdx.x;
-  const long long nt = (long long)gridDim.x * BS;
-  for (long long w = gt; w < a.nwords; w += nt) {
-    unsigned long long x = fr[w];
-    while (x) {
-      const int bp = __ffsll((long long)x) - 1;
-      x &= x - 1;
-      const int v = int(w * 64 + bp);
-      push_vertex(a, sm, v, a.off[v], a.off[v + 1], par, slot);
-    }
-  }
-  flush_push(a, sm, par, slot, false);
-}
-
-// Chooses the next phase. Runs on one thread after every block finished the
-// current phase.
+// Chooses the next phase. Runs on one thread after every block finished.
 __device__ void decide(const Args &a) {
   Control *c = a.ctl;
   const int L = c->level, ph = c->phase;
   c->phases += 1;
-  if (ph == PH_INSPECT) {
-    c->phase = PH_TD;
+  if (ph == PH_TD) {
+    c->phase = PH_INSPECT;
     return;
   }
-  LevelSlot *nx = &c->slot[(L + 1) % 3];
-  const unsigned long long nf = ld_agent(&nx->nf);
-  const unsigned long long mf = ld_agent(&nx->mf);
+  const Counters &nx = c->cnt[(L + 1) % 3];
+  unsigned long long nf = 0, mf = 0;
+  for (int s = 0; s < kStripes; ++s) {
+    nf += ld_agent(&nx.v[s * kStripeU64]);
+    mf += ld_agent(&nx.v[s * kStripeU64 + 1]);
+  }
+  Counters &z = c->cnt[(L + 2) % 3];
+  for (int s = 0; s < kStripes; ++s) {
+    st_agent(&z.v[s * kStripeU64], 0ull);
+    st_agent(&z.v[s * kStripeU64 + 1], 0ull);
+  }
   if (ph == PH_BU) {
     c->bu_levels += 1;
     if (L < 64) c->bu_mask |= 1ull << L;
@@ -401,17 +395,13 @@ __device__ void decide(const Args &a) {
   }
   c->total_nf += nf;
   c->total_mf += mf;
-  LevelSlot *z = &c->slot[(L + 2) % 3];
-  for (int i = 0; i < 4; ++i) st_agent(&z->qsize[i], 0u);
-  st_agent(&z->nf, 0u);
-  st_agent(&z->mf, 0ull);
   if (nf == 0) {
     c->levels = L + 1;
     c->done = 1;
     return;
   }
   if (L + 1 >= a.n) {
-    atomicOr(&c->error, ERR_LEVELS);
+    c->error |= ERR_LEVELS;
     c->done = 1;
     return;
   }
@@ -429,59 +419,58 @@ __device__ void decide(const Args &a) {
   }
   c->prev_nf = nf;
   c->level = L + 1;
-  if (next_bu) {
-    c->phase = PH_BU;
-    c->bitmap_valid = cur_bu ? 1 : 0;
-  } else {
-    c->phase = cur_bu ? PH_INSPECT : PH_TD;
-    c->bitmap_valid = 0;
-  }
+  c->phase = next_bu ? PH_BU : PH_TD;
 }
 
-// depth = -1, visited = isolated vertices and padding bits, root at level 0,
-// root queued in its class, counters cleared.
+// Two-level arrival: blocks count on their stripe, the last of each stripe on
+// arrive_all; returns true for the single last block. Thread 0 only.
+__device__ __forceinline__ bool arrive_last(Control *c) {
+  const unsigned G = gridDim.x;
+  const unsigned s = blockIdx.x % kStripes;
+  const unsigned quota = (G - s + kStripes - 1) / kStripes;
+  const unsigned nstripes = G < kStripes ? G : kStripes;
+  unsigned *sc = &c->arrive[s * 32];
+  const unsigned t = __hip_atomic_fetch_add(sc, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (t != quota - 1) return false;
+  st_agent(sc, 0u);
+  const unsigned t2 = __hip_atomic_fetch_add(&c->arrive_all, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (t2 != nstripes - 1) return false;
+  st_agent(&c->arrive_all, 0u);
+  return true;
+}
+
+// depth = -1 (root 0), visited = isolated vertices, padding bits and the root,
+// front[0] = {root}, marks and counters cleared.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_phase) {
   const long long gt = (long long)blockIdx.x * BS + threadIdx.x;
   const long long nt = (long long)gridDim.x * BS;
-  for (long long v = gt; v < a.n; v += nt) a.depth[v] = v == root ? 0 : -1;
-  for (long long w = gt; w < a.nwords; w += nt) {
-    unsigned long long vis = ~a.nzmask[w];
-    if (w == (root >> 6)) vis |= 1ull << (root & 63);
-    a.visited[w] = vis;
+  for (long long v = gt; v < a.n; v += nt) {
+    a.depth[v] = v == root ? 0 : -1;
+    a.mark[v] = 0;
   }
+  for (long long w = gt; w < a.nwords; w += nt) {
+    const unsigned long long rb = (w == (root >> 6)) ? 1ull << (root & 63) : 0ull;
+    a.visited[w] = ~a.nzmask[w] | rb;
+    a.front[0][w] = rb;
+  }
+  Control *c = a.ctl;
+  constexpr long long kC = (long long)kStripes * kStripeU64;
+  for (long long i = gt; i < 3 * kC; i += nt) c->cnt[i / kC].v[i % kC] = 0;
+  for (long long i = gt; i < kStripes * 32; i += nt) c->arrive[i] = 0;
   if (gt == 0) {
-    Control *c = a.ctl;
     c->level = 0;
     c->phase = first_phase;
     c->done = 0;
     c->error = 0;
-    c->bitmap_valid = 0;
     c->levels = 0;
     c->td_levels = c->bu_levels = c->phases = 0;
-    c->arrive = 0;
+    c->arrive_all = 0;
     c->gen = 0;
     c->prev_nf = 0;
     c->bu_mask = 0;
-    for (int s = 0; s < 3; ++s) {
-      for (int i = 0; i < 4; ++i) c->slot[s].qsize[i] = 0;
-      c->slot[s].nf = 0;
-      c->slot[s].mf = 0;
-    }
-    const long long s0 = a.off[root], e0 = a.off[root + 1], d = e0 - s0;
     c->total_nf = 1;
-    c->total_mf = (unsigned long long)d;
-    c->slot[0].nf = 1;
-    c->slot[0].mf = (unsigned long long)d;
-    if (d >= a.t_grid) {
-      const unsigned np = unsigned((d + a.piece - 1) / a.piece);
-      for (unsigned k = 0; k < np && k < a.cap3; ++k) a.qg[0][k] = ((long long)root << 32) | k;
-      c->slot[0].qsize[3] = np;
-    } else {
-      const int cls = d >= a.t_block ? 2 : (d >= a.t_wave ? 1 : 0);
-      a.q[0][cls][0] = root;
-      c->slot[0].qsize[cls] = 1;
-    }
+    c->total_mf = (unsigned long long)(a.off[root + 1] - a.off[root]);
   }
 }
 
@@ -492,8 +481,6 @@ __device__ __forceinline__ void load_control(const Args &a, Smem<BS> &sm) {
     sm.L = ld_agent(&c->level);
     sm.ph = ld_agent(&c->phase);
     sm.done = ld_agent(&c->done);
-    sm.bm = ld_agent(&c->bitmap_valid);
-    sm.cnt[0] = sm.cnt[1] = sm.cnt[2] = 0;
     sm.nf = sm.mf = 0;
   }
   __syncthreads();
@@ -502,7 +489,7 @@ __device__ __forceinline__ void load_control(const Args &a, Smem<BS> &sm) {
 template <int BS>
 __device__ __forceinline__ void run_phase(const Args &a, Smem<BS> &sm) {
   if (sm.ph == PH_TD) td_phase<BS>(a, sm, sm.L);
-  else if (sm.ph == PH_BU) bu_phase<BS>(a, sm, sm.L, sm.bm != 0);
+  else if (sm.ph == PH_BU) bu_phase<BS>(a, sm, sm.L);
   else inspect_phase<BS>(a, sm, sm.L);
 }
 
@@ -516,11 +503,9 @@ __global__ __launch_bounds__(BS) void k_level(Args a) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned t = atomicAdd(&a.ctl->arrive, 1u);
-    if (t == gridDim.x - 1) {
+    if (arrive_last(a.ctl)) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       decide(a);
-      st_agent(&a.ctl->arrive, 0u);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     }
   }
@@ -535,15 +520,14 @@ __device__ __forceinline__ void grid_barrier_decide(const Args &a) {
   if (threadIdx.x == 0) {
     Control *c = a.ctl;
     const unsigned g = ld_agent(&c->gen);
-    const unsigned t = __hip_atomic_fetch_add(&c->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (t == gridDim.x - 1) {
+    if (arrive_last(c)) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       decide(a);
-      st_agent(&c->arrive, 0u);
       __hip_atomic_store(&c->gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       const uint64_t t0 = rt_clock();
       while (__hip_atomic_load(&c->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(1);
         if (rt_clock() - t0 > 400000000ull) {
           atomicOr(&c->error, ERR_TIMEOUT);
           st_agent(&c->done, 1);
@@ -558,7 +542,7 @@ __device__ __forceinline__ void grid_barrier_decide(const Args &a) {
 
 // Whole search in one cooperative launch.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_persist(Args a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_persist(Args a) {
   __shared__ Smem<BS> sm;
   for (;;) {
     load_control(a, sm);
@@ -595,12 +579,16 @@ struct ent_graph {
   long long *off = nullptr;
   int *adj = nullptr;
   unsigned long long *nzmask = nullptr;
+  int *hub_block = nullptr;
+  int *piece_v = nullptr;
+  int *piece_k = nullptr;
+  int n_hub_block = 0, n_pieces = 0;
+  int lists_t_block = -1, lists_t_grid = -1, lists_piece = -1;
+  std::vector<int32_t> h_deg;  // host copy of degrees (for the static lists)
   int *depth = nullptr;
   unsigned long long *visited = nullptr;
   unsigned long long *front[2] = {nullptr, nullptr};
-  int *q[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-  long long *qg[2] = {nullptr, nullptr};
-  int64_t cap3 = 0;
+  unsigned char *mark = nullptr;
   Control *ctl = nullptr;
   Control *h_ctl = nullptr;
   ent_bfs_config cfg{};
@@ -624,20 +612,26 @@ struct ent_graph {
 
 namespace {
 
+template <class T>
+inline void ignore(T) {}
+
 int dev_alloc(ent_graph *g, void **p, size_t bytes) {
   if (bytes == 0) bytes = 8;
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
-    (void)hipGetLastError();
+    ignore(hipGetLastError());
     return set_error(ENT_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
   }
   g->dev_bytes += bytes;
   return ENT_OK;
 }
 
-template <class T>
-inline void ignore(T) {}
+void dev_free(ent_graph *g, void *p, size_t bytes) {
+  if (!p) return;
+  ignore(hipFree(p));
+  g->dev_bytes -= std::min(g->dev_bytes, bytes);
+}
 
 void release(ent_graph *g) {
   if (!g) return;
@@ -647,13 +641,14 @@ void release(ent_graph *g) {
   ignore(hipFree(g->off));
   ignore(hipFree(g->adj));
   ignore(hipFree(g->nzmask));
+  ignore(hipFree(g->hub_block));
+  ignore(hipFree(g->piece_v));
+  ignore(hipFree(g->piece_k));
   ignore(hipFree(g->depth));
   ignore(hipFree(g->visited));
-  for (int p = 0; p < 2; ++p) {
-    ignore(hipFree(g->front[p]));
-    for (int c = 0; c < 3; ++c) ignore(hipFree(g->q[p][c]));
-    ignore(hipFree(g->qg[p]));
-  }
+  ignore(hipFree(g->front[0]));
+  ignore(hipFree(g->front[1]));
+  ignore(hipFree(g->mark));
   ignore(hipFree(g->ctl));
   if (g->h_ctl) ignore(hipHostFree(g->h_ctl));
   if (g->ev0) ignore(hipEventDestroy(g->ev0));
@@ -662,8 +657,47 @@ void release(ent_graph *g) {
   delete g;
 }
 
-int64_t piece_capacity(int64_t n, int64_t m, int t_grid, int piece) {
-  return std::min<int64_t>(n, m / std::max(1, t_grid) + 1) + m / std::max(1, piece) + 1;
+// Rebuilds the static block-class hub list and grid-class piece table for the
+// thresholds in g->cfg (degrees from the host copy).
+int build_lists(ent_graph *g) {
+  const ent_bfs_config &c = g->cfg;
+  if (g->lists_t_block == c.t_block && g->lists_t_grid == c.t_grid && g->lists_piece == c.piece) return ENT_OK;
+  std::vector<int32_t> hubs, pv, pk;
+  for (int64_t v = 0; v < g->n; ++v) {
+    const int32_t d = g->h_deg[size_t(v)];
+    if (d >= c.t_grid) {
+      const int32_t np = int32_t((int64_t(d) + c.piece - 1) / c.piece);
+      for (int32_t k = 0; k < np; ++k) {
+        pv.push_back(int32_t(v));
+        pk.push_back(k);
+      }
+    } else if (d >= c.t_block) {
+      hubs.push_back(int32_t(v));
+    }
+  }
+  if (hubs.size() > size_t(INT32_MAX) || pv.size() > size_t(INT32_MAX))
+    return set_error(ENT_EINVAL, "static class lists too long");
+  dev_free(g, g->hub_block, sizeof(int) * size_t(std::max(g->n_hub_block, 1)));
+  dev_free(g, g->piece_v, sizeof(int) * size_t(std::max(g->n_pieces, 1)));
+  dev_free(g, g->piece_k, sizeof(int) * size_t(std::max(g->n_pieces, 1)));
+  g->hub_block = g->piece_v = g->piece_k = nullptr;
+  g->n_hub_block = g->n_pieces = 0;
+  g->lists_t_block = g->lists_t_grid = g->lists_piece = -1;
+  int rc;
+  if ((rc = dev_alloc(g, reinterpret_cast<void **>(&g->hub_block), sizeof(int) * std::max<size_t>(hubs.size(), 1)))) return rc;
+  if ((rc = dev_alloc(g, reinterpret_cast<void **>(&g->piece_v), sizeof(int) * std::max<size_t>(pv.size(), 1)))) return rc;
+  if ((rc = dev_alloc(g, reinterpret_cast<void **>(&g->piece_k), sizeof(int) * std::max<size_t>(pk.size(), 1)))) return rc;
+  if (!hubs.empty()) HIP_TRY(hipMemcpy(g->hub_block, hubs.data(), sizeof(int) * hubs.size(), hipMemcpyHostToDevice));
+  if (!pv.empty()) {
+    HIP_TRY(hipMemcpy(g->piece_v, pv.data(), sizeof(int) * pv.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->piece_k, pk.data(), sizeof(int) * pk.size(), hipMemcpyHostToDevice));
+  }
+  g->n_hub_block = int(hubs.size());
+  g->n_pieces = int(pv.size());
+  g->lists_t_block = c.t_block;
+  g->lists_t_grid = c.t_grid;
+  g->lists_piece = c.piece;
+  return ENT_OK;
 }
 
 int check_config(const ent_bfs_config *c) {
@@ -688,51 +722,51 @@ Args make_args(ent_graph *g, int *depth) {
   a.off = g->off;
   a.adj = g->adj;
   a.nzmask = g->nzmask;
+  a.hub_block = g->hub_block;
+  a.piece_v = g->piece_v;
+  a.piece_k = g->piece_k;
   a.depth = depth;
   a.visited = g->visited;
   a.front[0] = g->front[0];
   a.front[1] = g->front[1];
-  for (int p = 0; p < 2; ++p) {
-    for (int c = 0; c < 3; ++c) a.q[p][c] = g->q[p][c];
-    a.qg[p] = g->qg[p];
-  }
+  a.mark = g->mark;
   a.ctl = g->ctl;
   a.m = g->m;
   a.n_active = g->n_active;
-  a.cap3 = g->cap3;
   a.n = int(g->n);
   a.nwords = int(g->nwords);
+  a.n_hub_block = g->n_hub_block;
+  a.n_pieces = g->n_pieces;
   const ent_bfs_config &c = g->cfg;
   a.t_wave = c.t_wave;
   a.t_block = c.t_block;
-  a.t_grid = c.t_grid;
   a.piece = c.piece;
   a.mode = c.direction;
   a.alpha = c.alpha;
   a.beta = c.beta;
   a.bu_probe = c.bu_probe;
-  a.bu_chunk_words = c.bu_chunk_words;
+  a.chunk_words = c.bu_chunk_words;
   return a;
 }
 
-template <int BS>
-struct Kernels {
-  static const void *init() { return reinterpret_cast<const void *>(&k_init<BS>); }
-  static const void *level() { return reinterpret_cast<const void *>(&k_level<BS>); }
-  static const void *persist() { return reinterpret_cast<const void *>(&k_persist<BS>); }
-};
-
 struct KernelSet {
   const void *init, *level, *persist;
+  size_t smem;
 };
+
+template <int BS>
+KernelSet kernel_set() {
+  return {reinterpret_cast<const void *>(&k_init<BS>), reinterpret_cast<const void *>(&k_level<BS>),
+          reinterpret_cast<const void *>(&k_persist<BS>), sizeof(Smem<BS>)};
+}
 
 KernelSet kernels_for(int bs) {
   switch (bs) {
-    case 64: return {Kernels<64>::init(), Kernels<64>::level(), Kernels<64>::persist()};
-    case 128: return {Kernels<128>::init(), Kernels<128>::level(), Kernels<128>::persist()};
-    case 512: return {Kernels<512>::init(), Kernels<512>::level(), Kernels<512>::persist()};
-    case 1024: return {Kernels<1024>::init(), Kernels<1024>::level(), Kernels<1024>::persist()};
-    default: return {Kernels<256>::init(), Kernels<256>::level(), Kernels<256>::persist()};
+    case 64: return kernel_set<64>();
+    case 128: return kernel_set<128>();
+    case 512: return kernel_set<512>();
+    case 1024: return kernel_set<1024>();
+    default: return kernel_set<256>();
   }
 }
 
@@ -741,12 +775,32 @@ int level_grid(ent_graph *g) {
   return g->cus * bpc;
 }
 
+// Blocks per CU for the cooperative kernel. Every block must be resident at
+// once, and the runtime's occupancy query does not see every register limit,
+// so the grid stays at or below 16 wavefronts and 8 workgroups per CU (the
+// kernel is compiled for at least 4 wavefronts per SIMD), within the LDS
+// budget (160 KiB per CU, 2 KiB allocation granule) and blocks_per_cu.
+int persistent_blocks_per_cu(ent_graph *g, const KernelSet &ks) {
+  const int bs = g->cfg.block_size;
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ks.persist, bs, 0) != hipSuccess) {
+    ignore(hipGetLastError());
+    return 0;
+  }
+  const size_t lds_block = (ks.smem + 2047) / 2048 * 2048;
+  const int lds_limit = int((160 * 1024) / lds_block);
+  const int wave_limit = std::max(1, 16 / (bs / kWave));
+  int per_cu = std::min(std::min(occ, lds_limit), std::min(wave_limit, 8));
+  if (g->cfg.blocks_per_cu > 0) per_cu = std::min(per_cu, g->cfg.blocks_per_cu);
+  return per_cu;
+}
+
 hipError_t launch(const void *fn, int grid, int bs, void **args, hipStream_t s) {
   return hipLaunchKernel(fn, dim3(grid), dim3(bs), args, 0, s);
 }
 
 // Queues the per-phase kernel k times (used directly and under capture).
-hipError_t queue_levels(ent_graph *g, const KernelSet &ks, Args *a, int k) {
+hipError_t queue_phases(ent_graph *g, const KernelSet &ks, Args *a, int k) {
   void *args[] = {a};
   const int grid = level_grid(g);
   g->last_grid = grid;
@@ -757,43 +811,16 @@ hipError_t queue_levels(ent_graph *g, const KernelSet &ks, Args *a, int k) {
   return hipSuccess;
 }
 
-size_t smem_bytes(int bs) {
-  switch (bs) {
-    case 64: return sizeof(Smem<64>);
-    case 128: return sizeof(Smem<128>);
-    case 512: return sizeof(Smem<512>);
-    case 1024: return sizeof(Smem<1024>);
-    default: return sizeof(Smem<256>);
-  }
-}
-
-// Blocks per CU for the cooperative kernel: the occupancy query, capped by
-// the LDS budget (160 KiB per CU, each block's LDS rounded up to a 2 KiB
-// allocation granule), 32 wavefronts per CU and 16 workgroups per CU, and by
-// blocks_per_cu when set. Every block must be resident at once.
-int persistent_blocks_per_cu(ent_graph *g, const KernelSet &ks) {
-  const int bs = g->cfg.block_size;
-  int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ks.persist, bs, 0) != hipSuccess) {
-    (void)hipGetLastError();
-    return 0;
-  }
-  const size_t lds_block = (smem_bytes(bs) + 2047) / 2048 * 2048;
-  const int lds_limit = int((160 * 1024) / lds_block);
-  const int wave_limit = 32 / (bs / kWave);
-  int per_cu = std::min(std::min(occ, lds_limit), std::min(wave_limit, 16));
-  if (g->cfg.blocks_per_cu > 0) per_cu = std::min(per_cu, g->cfg.blocks_per_cu);
-  return per_cu;
-}
-
 int run_search(ent_graph *g, int root, int *depth_dev) {
   const ent_bfs_config &cfg = g->cfg;
+  int rc = build_lists(g);
+  if (rc) return rc;
   const KernelSet ks = kernels_for(cfg.block_size);
   Args a = make_args(g, depth_dev);
   const int bs = cfg.block_size;
-  const int first_phase = cfg.direction == ENT_DIR_BOTTOMUP ? PH_BU : PH_TD;
+  int first_phase = cfg.direction == ENT_DIR_BOTTOMUP ? PH_BU : PH_TD;
   {
-    void *args[] = {&a, &root, const_cast<int *>(&first_phase)};
+    void *args[] = {&a, &root, &first_phase};
     const int grid = int(std::min<int64_t>(int64_t(g->cus) * 8, (g->n + bs - 1) / bs + 1));
     HIP_TRY(launch(ks.init, std::max(1, grid), bs, args, g->stream));
   }
@@ -811,7 +838,7 @@ int run_search(ent_graph *g, int root, int *depth_dev) {
     case ENT_LAUNCH_STREAM: {
       const int k = cfg.launch == ENT_LAUNCH_LEVELS ? 1 : cfg.graph_levels;
       for (int64_t ph = 0;; ph += k) {
-        HIP_TRY(queue_levels(g, ks, &a, k));
+        HIP_TRY(queue_phases(g, ks, &a, k));
         HIP_TRY(hipMemcpyAsync(g->h_ctl, g->ctl, 16, hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
         if (g->h_ctl->done) break;
@@ -829,7 +856,7 @@ int run_search(ent_graph *g, int root, int *depth_dev) {
         g->gexec = nullptr;
         g->graph = nullptr;
         HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = queue_levels(g, ks, &a, k);
+        hipError_t e = queue_phases(g, ks, &a, k);
         hipGraph_t gr = nullptr;
         hipError_t e2 = hipStreamEndCapture(g->stream, &gr);
         if (e != hipSuccess || e2 != hipSuccess)
@@ -841,6 +868,7 @@ int run_search(ent_graph *g, int root, int *depth_dev) {
         g->graph_grid = grid;
         g->graph_k = k;
       }
+      g->last_grid = grid;
       for (int64_t ph = 0;; ph += k) {
         HIP_TRY(hipGraphLaunch(g->gexec, g->stream));
         HIP_TRY(hipMemcpyAsync(g->h_ctl, g->ctl, 16, hipMemcpyDeviceToHost, g->stream));
@@ -888,16 +916,16 @@ int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int6
     return set_error(ENT_EINVAL, "offsets do not span [0, m=%lld]", (long long)m);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    (void)hipGetLastError();
+    ignore(hipGetLastError());
     return set_error(ENT_EDEVICE, "no HIP device available");
   }
   if (device < 0 || device >= ndev) return set_error(ENT_EINVAL, "device %d of %d", device, ndev);
-  // Host-side structural checks: monotone offsets and in-range ids.
+  // Host-side structural checks: monotone offsets, degrees < 2^31, in-range ids.
   {
     std::atomic<int64_t> bad{-1};
     parallel_for(n, 1 << 15, [&](int64_t b, int64_t e) {
       for (int64_t v = b; v < e; ++v) {
-        if (beg_pos[v + 1] < beg_pos[v]) { bad.store(v); return; }
+        if (beg_pos[v + 1] < beg_pos[v] || beg_pos[v + 1] - beg_pos[v] > INT32_MAX) { bad.store(v); return; }
         for (int64_t i = beg_pos[v]; i < beg_pos[v + 1]; ++i)
           if (adj[i] < 0 || adj[i] >= n) { bad.store(v); return; }
       }
@@ -935,24 +963,25 @@ int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int6
   UP_TRY(hipEventCreate(&g->ev1));
   UP_TRY(hipDeviceGetAttribute(&g->cus, hipDeviceAttributeMultiprocessorCount, device));
   std::vector<unsigned long long> nz(size_t(g->nwords), 0ull);
+  g->h_deg.resize(size_t(n));
   int64_t active = 0;
-  for (int64_t v = 0; v < n; ++v)
-    if (beg_pos[v + 1] > beg_pos[v]) {
+  for (int64_t v = 0; v < n; ++v) {
+    const int64_t d = beg_pos[v + 1] - beg_pos[v];
+    g->h_deg[size_t(v)] = int32_t(d);
+    if (d > 0) {
       nz[size_t(v >> 6)] |= 1ull << (v & 63);
       ++active;
     }
+  }
   g->n_active = active;
-  g->cap3 = piece_capacity(n, m, g->cfg.t_grid, g->cfg.piece);
   UP_ALLOC(g->off, sizeof(long long) * size_t(n + 1));
   UP_ALLOC(g->adj, sizeof(int) * size_t(m));
   UP_ALLOC(g->nzmask, sizeof(unsigned long long) * size_t(g->nwords));
   UP_ALLOC(g->depth, sizeof(int) * size_t(n));
   UP_ALLOC(g->visited, sizeof(unsigned long long) * size_t(g->nwords));
-  for (int p = 0; p < 2; ++p) {
-    UP_ALLOC(g->front[p], sizeof(unsigned long long) * size_t(g->nwords));
-    for (int c = 0; c < 3; ++c) UP_ALLOC(g->q[p][c], sizeof(int) * size_t(n));
-    UP_ALLOC(g->qg[p], sizeof(long long) * size_t(g->cap3));
-  }
+  UP_ALLOC(g->front[0], sizeof(unsigned long long) * size_t(g->nwords));
+  UP_ALLOC(g->front[1], sizeof(unsigned long long) * size_t(g->nwords));
+  UP_ALLOC(g->mark, size_t(n));
   UP_ALLOC(g->ctl, sizeof(Control));
   UP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_ctl), sizeof(Control), hipHostMallocDefault));
   UP_TRY(hipMemcpy(g->off, beg_pos, sizeof(long long) * size_t(n + 1), hipMemcpyHostToDevice));
@@ -962,6 +991,11 @@ int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int6
   UP_TRY(hipDeviceSynchronize());
 #undef UP_TRY
 #undef UP_ALLOC
+  rc = build_lists(g);
+  if (rc) {
+    release(g);
+    return rc;
+  }
   *out = g;
   return ENT_OK;
 }
@@ -994,20 +1028,10 @@ int ent_set_config(ent_graph *g, const ent_bfs_config *cfg) {
   int rc = check_config(cfg);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(g->mu);
-  const int64_t need = piece_capacity(g->n, g->m, cfg->t_grid, cfg->piece);
-  if (need > g->cap3) {
-    HIP_TRY(hipSetDevice(g->device));
-    HIP_TRY(hipStreamSynchronize(g->stream));
-    for (int p = 0; p < 2; ++p) {
-      ignore(hipFree(g->qg[p]));
-      g->qg[p] = nullptr;
-      rc = dev_alloc(g, reinterpret_cast<void **>(&g->qg[p]), sizeof(long long) * size_t(need));
-      if (rc) return rc;
-    }
-    g->cap3 = need;
-  }
+  HIP_TRY(hipSetDevice(g->device));
+  HIP_TRY(hipStreamSynchronize(g->stream));
   g->cfg = *cfg;
-  return ENT_OK;
+  return build_lists(g);
 }
 
 int ent_get_config(const ent_graph *g, ent_bfs_config *cfg) {
@@ -1034,13 +1058,12 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
     return rc;
   }
   HIP_TRY(hipEventRecord(g->ev1, g->stream));
-  HIP_TRY(hipMemcpyAsync(g->h_ctl, g->ctl, sizeof(Control), hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(g->h_ctl, g->ctl, offsetof(Control, arrive_all), hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
   const auto t1 = std::chrono::steady_clock::now();
   const Control &c = *g->h_ctl;
   if (c.error)
-    return set_error(ENT_EINTERNAL, "device reported error flags 0x%x (1 queue overflow, 2 level guard, 4 barrier timeout)",
-                     c.error);
+    return set_error(ENT_EINTERNAL, "device reported error flags 0x%x (2 level guard, 4 barrier timeout)", c.error);
   if (!c.done) return set_error(ENT_EINTERNAL, "search ended without completion");
   if (st) {
     std::memset(st, 0, sizeof *st);
