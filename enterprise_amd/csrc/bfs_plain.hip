@@ -914,12 +914,7 @@ int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int6
   if (n < 1 || n >= (int64_t(1) << 31) - 64) return set_error(ENT_EINVAL, "n=%lld outside [1, 2^31-64)", (long long)n);
   if (m < 0 || beg_pos[0] != 0 || beg_pos[n] != m)
     return set_error(ENT_EINVAL, "offsets do not span [0, m=%lld]", (long long)m);
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    ignore(hipGetLastError());
-    return set_error(ENT_EDEVICE, "no HIP device available");
-  }
-  if (device < 0 || device >= ndev) return set_error(ENT_EINVAL, "device %d of %d", device, ndev);
+  if (device < 0) return set_error(ENT_EINVAL, "device %d", device);
   // Host-side structural checks: monotone offsets, degrees < 2^31, in-range ids.
   {
     std::atomic<int64_t> bad{-1};
@@ -932,6 +927,12 @@ int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int6
     });
     if (bad.load() >= 0) return set_error(ENT_EINVAL, "malformed CSR at vertex %lld", (long long)bad.load());
   }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    ignore(hipGetLastError());
+    return set_error(ENT_EDEVICE, "no HIP device available");
+  }
+  if (device >= ndev) return set_error(ENT_EINVAL, "device %d of %d", device, ndev);
   ent_graph *g = new (std::nothrow) ent_graph();
   if (!g) return set_error(ENT_ENOMEM, "out of host memory");
   g->device = device;
