@@ -81,7 +81,7 @@ def load_tuned(dg, name):
     if os.path.exists(path):
         with open(path) as f:
             cfg = json.load(f)
-        dg.configure(**{k: v for k, v in cfg.get("config", cfg).items() if k != "reserved"})
+        dg.configure(**{k: v for k, v in cfg.get("config", cfg).items() if k not in ("reserved", "launch")})
         return path
     return None
 
@@ -181,7 +181,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=8)
-    ap.add_argument("--mode", default="plain", choices=["plain", "grid", "batch", "serve"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "grid", "batch", "serve", "calib"])
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--launch", default=None, choices=["persistent", "levels", "graph", "stream"])
     ap.add_argument("--no-cpu", action="store_true")
@@ -196,8 +196,8 @@ def main():
     if args.mode == "plain":
         out = bench_plain(args)
     else:
-        import bench_modes
-        out = bench_modes.run(args)
+        from enterprise_amd import benchmarks
+        out = benchmarks.run(args)
     print(json.dumps(out), flush=True)
 
 

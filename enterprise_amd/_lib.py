@@ -39,7 +39,7 @@ class BfsConfig(C.Structure):
     _fields_ = [(name, C.c_int32) for name in (
         "direction", "launch", "alpha", "beta", "t_wave", "t_block", "t_grid",
         "piece", "block_size", "blocks_per_cu", "bu_probe", "bu_chunk_words",
-        "graph_levels", "hub_cache")] + [("reserved", C.c_int32 * 2)]
+        "graph_levels", "hub_cache", "batch_alpha")] + [("reserved", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
@@ -58,14 +58,14 @@ class Stats(C.Structure):
 
 
 class BatchStats(C.Structure):
-    _fields_ = [("k", C.c_int32), ("groups", C.c_int32), ("levels", C.c_int32),
-                ("reserved", C.c_int32),
+    _fields_ = [("k", C.c_int32), ("words", C.c_int32), ("levels", C.c_int32),
+                ("phases", C.c_int32),
                 ("reached_edges", C.c_int64), ("reached_vertices", C.c_int64),
                 ("alg_bytes", C.c_int64), ("device_ms", C.c_double),
-                ("host_ms", C.c_double)]
+                ("host_ms", C.c_double), ("bu_level_mask", C.c_uint64)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 # (name, restype, argtypes) for every exported function.
@@ -100,6 +100,17 @@ SIGNATURES = [
     ("ent_bfs_gpu_coalescing_mem", C.c_int, [i32p, C.c_int32, i64p, i32p,
                                              C.c_int64, C.c_int64, C.c_int32]),
     ("ent_selftest_wave", C.c_int, [C.c_int]),
+    ("ent_calib_copy", C.c_int, [C.c_int, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    ("ent_batch_reserve", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("ent_batch_release", C.c_int, [C.c_void_p]),
+    ("ent_batch_workspace_bytes", C.c_int64, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("ent_bfs_batch_device", C.c_int, [C.c_void_p, i32p, C.c_int32, C.c_void_p,
+                                       C.POINTER(BatchStats)]),
+    ("ent_bfs_batch", C.c_int, [C.c_void_p, i32p, C.c_int32, i32p, C.POINTER(BatchStats)]),
+    ("ent_device_alloc", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),
+    ("ent_device_free", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("ent_device_row_hash", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
+                                      C.POINTER(C.c_uint64)]),
 ]
 
 _lib = None

@@ -1,0 +1,107 @@
+"""Benchmark modes other than the default plain BFS line of bench.py."""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+from ._lib import check, lib
+
+
+def calib(args):
+    """Copy kernel of known size, for scaling hardware byte counters."""
+    nbytes = 1 << 30
+    ms = C.c_double()
+    check(lib().ent_calib_copy(0, nbytes, 5, C.byref(ms)))
+    return {"mode": "calib", "bytes_read": nbytes, "bytes_written": nbytes,
+            "ms_per_copy": round(ms.value, 4),
+            "GBps_read_plus_write": round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)}
+
+
+MODES = {"calib": calib}
+
+
+def run(args):
+    t0 = time.perf_counter()
+    out = MODES[args.mode](args)
+    out.setdefault("wall_s", round(time.perf_counter() - t0, 2))
+    return out
+
+
+def _hmean(xs):
+    xs = [x for x in xs if x > 0]
+    return len(xs) / sum(1.0 / x for x in xs) if xs else 0.0
+
+
+def batch(args):
+    """Config 4: 512 roots on R-MAT scale 23 (half from the top 1% degree
+    vertices, half of degree 1..4) in one multi-source search, against the
+    plain single-source path on the same roots. Every batch row is compared
+    with the plain result by a device-side row checksum."""
+    import sys
+
+    import numpy as np
+
+    from .bfs import DeviceGraph
+    from .graph import HostGraph
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    scale = getattr(args, "scale", 23) if getattr(args, "scale", 22) != 22 else 23
+    k = 512
+    t0 = time.perf_counter()
+    g = HostGraph.rmat(scale, 16, seed=scale)
+    roots = g.pick_roots_mixed(k, k // 2, 1004)
+    log(f"[batch] R-MAT scale {scale}: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f}s)")
+    dg = DeviceGraph(g)
+    from . import autotune
+    tuned = autotune.load(f"rmat{scale}") or autotune.load("rmat22")
+    if tuned:
+        dg.configure(**{kk: v for kk, v in tuned.items() if kk not in ("reserved", "launch")})
+    # Plain path on the same roots (host window per root, as bench.py plain).
+    plain_rates, plain_edges, plain_time, plain_hash = [], 0, 0.0, []
+    for r in roots[:8]:
+        dg.bfs_device(int(r))
+    for r in roots:
+        t1 = time.perf_counter()
+        st = dg.bfs_device(int(r))
+        dt = time.perf_counter() - t1
+        plain_edges += st["reached_edges"]
+        plain_time += dt
+        if st["levels"] > 3:
+            plain_rates.append(st["reached_edges"] / dt / 1e9)
+        plain_hash.append(int(dg.row_hash(dg.depth_device_ptr(), 1, g.n)[0]))
+    log(f"[batch] plain: {len(roots)} roots in {plain_time * 1e3:.1f} ms")
+    out = dg.device_alloc(k * g.n * 4)
+    try:
+        dg.batch_reserve(k, 48)
+        dg.bfs_batch_device(roots[:64], out)            # warm-up
+        stats = []
+        for _ in range(max(1, args.steps if args.steps < 64 else 3)):
+            stats.append(dg.bfs_batch_device(roots, out))
+        h = dg.row_hash(out, k, g.n)
+        mism = int(sum(int(h[i]) != plain_hash[i] for i in range(k)))
+    finally:
+        dg.device_free(out)
+    best = min(stats, key=lambda s: s["host_ms"])
+    batch_gteps = best["reached_edges"] / (best["host_ms"] * 1e-3) / 1e9
+    plain_agg = plain_edges / plain_time / 1e9
+    achieved = best["alg_bytes"] / (best["device_ms"] * 1e-3) / 1e9
+    dg.free()
+    return {
+        "mode": "batch", "metric": "aggregate GTEPS, 512-source batched vs plain",
+        "value": round(batch_gteps, 2), "unit": "GTEPS", "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"multi-source BFS, R-MAT scale {scale} ef 16 (seed {scale}), 512 roots "
+                               "(256 top-1% degree, 256 degree 1..4, seed 1004)", "k": k, "words": best["words"]},
+        "batch_ms": round(best["host_ms"], 3), "batch_device_ms": round(best["device_ms"], 3),
+        "plain_aggregate_gteps": round(plain_agg, 2), "plain_hmean_gteps": round(_hmean(plain_rates), 2),
+        "plain_total_ms": round(plain_time * 1e3, 2),
+        "speedup_vs_plain": round(batch_gteps / plain_agg, 2),
+        "rows_mismatching_plain": mism, "levels": best["levels"], "phases": best["phases"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "alg_bytes": best["alg_bytes"]},
+        "runs_ms": [round(s["host_ms"], 3) for s in stats],
+    }
+
+
+MODES["batch"] = batch

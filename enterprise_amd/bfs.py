@@ -12,7 +12,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import BfsConfig, Stats, check, i32p, i64p, lib
+from ._lib import BatchStats, BfsConfig, Stats, check, i32p, i64p, lib
 
 
 def default_config():
@@ -91,6 +91,45 @@ class DeviceGraph:
         check(lib().ent_copy_to_host(self.handle, C.c_void_p(src), out.ctypes.data_as(i32p), count))
         return out
 
+    # -- multi-source ------------------------------------------------------
+    def batch_reserve(self, k, max_levels=0):
+        check(lib().ent_batch_reserve(self.handle, int(k), int(max_levels)))
+
+    def batch_release(self):
+        check(lib().ent_batch_release(self.handle))
+
+    def bfs_batch(self, roots):
+        """Multi-source BFS; returns (depths int32[k, n], stats)."""
+        r = np.ascontiguousarray(roots, dtype=np.int32)
+        out = np.empty((r.size, self.n), dtype=np.int32)
+        st = BatchStats()
+        check(lib().ent_bfs_batch(self.handle, r.ctypes.data_as(i32p), int(r.size),
+                                  out.ctypes.data_as(i32p), C.byref(st)))
+        return out, st.as_dict()
+
+    def bfs_batch_device(self, roots, depths_dev):
+        """Multi-source BFS writing k*n int32 depths to device memory."""
+        r = np.ascontiguousarray(roots, dtype=np.int32)
+        st = BatchStats()
+        check(lib().ent_bfs_batch_device(self.handle, r.ctypes.data_as(i32p), int(r.size),
+                                         C.c_void_p(depths_dev), C.byref(st)))
+        return st.as_dict()
+
+    def device_alloc(self, nbytes):
+        p = C.c_void_p()
+        check(lib().ent_device_alloc(self.handle, int(nbytes), C.byref(p)))
+        return p.value
+
+    def device_free(self, ptr):
+        check(lib().ent_device_free(self.handle, C.c_void_p(ptr)))
+
+    def row_hash(self, dev_ptr, rows, cols):
+        """Position-keyed 64-bit checksum of each row of an int32 device array."""
+        out = np.empty(rows, dtype=np.uint64)
+        check(lib().ent_device_row_hash(self.handle, C.c_void_p(dev_ptr), int(rows), int(cols),
+                                        out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return out
+
     def free(self):
         if getattr(self, "_h", None):
             lib().ent_graph_free(self._h)
@@ -117,6 +156,19 @@ def bfs_gpu_coalescing_mem(src_list, beg_pos, csr, gpu_id=0):
     return check(lib().ent_bfs_gpu_coalescing_mem(
         s.ctypes.data_as(i32p), int(s.size), b.ctypes.data_as(i64p), a.ctypes.data_as(i32p),
         int(b.size - 1), int(a.size), int(gpu_id)))
+
+
+def row_hash_host(a):
+    """Host form of the device row checksum (sum over i of mix64((i << 32) ^ value))."""
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    idx = np.arange(a.shape[-1], dtype=np.uint64)
+    z = (idx << np.uint64(32)) ^ a.view(np.uint32).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+        return z.sum(axis=-1, dtype=np.uint64)
 
 
 def selftest_wave(device=0):

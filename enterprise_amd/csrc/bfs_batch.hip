@@ -1,0 +1,925 @@
+// Multi-source BFS (up to 512 roots per call) on a resident graph, behind
+// include/ent_batch.h. No reference counterpart (the reference runs one root
+// at a time, bfs_gpu_opt.cuh:370); each output row must equal the
+// single-source result for its root, which the tests check bit for bit.
+//
+// Layout (vertex-major masks, W u64 words per vertex, bit s%64 of word s/64
+// is source s):
+//   seen  u64[n * W]              sources that reached v
+//   lv    u64[(lcap + 3) * n * W] lv[L][v] = sources whose level-L frontier
+//                                 holds v (new bits only; kept for the output)
+//   jf[2] u64[ceil(n/64)]         joint frontier bitmaps (v in any frontier)
+// One launch per phase, with the same launch-indexed state scheme as the
+// single-source path (launch i reads the state and finished counters of
+// launch i - 1; no fences, no contended atomics):
+//   TD     push: every joint-frontier vertex u ORs its masks into lv[L+1][v]
+//          of its neighbours (minus seen) with fire-and-forget atomics;
+//   APPLY  after TD: seen |= lv[L+1], joint frontier bitmap, counts;
+//   BU     pull: every vertex with unseen bits of still-active sources ORs
+//          the level-L masks of its neighbours, stopping once every such
+//          source has been found (or the list ends);
+//   OUTPUT replays lv[0..levels) per 64-vertex segment in LDS and writes all
+//          k depth rows with full, coalesced 256-byte stores.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "ent_batch.h"
+#include "ent_bfs.h"
+#include "ent_common.h"
+#include "ent_device.h"
+#include "ent_internal.h"
+
+namespace ent {
+
+enum BPhase : int { BPH_TD = 0, BPH_BU = 1, BPH_APPLY = 2, BPH_OUTPUT = 3 };
+enum : int { BERR_LEVELS = 8 };
+
+constexpr int kBStripes = 64;
+constexpr int kBStripeU64 = 16;
+// Counter words within a stripe.
+enum : int { C_JWORDS = 0, C_MF = 1, C_ALG = 2, C_MUNF = 3, C_ACTIVE = 4, C_EDGES = 12, C_VERTS = 13 };
+
+struct BCounters {
+  unsigned long long v[kBStripes * kBStripeU64];
+};
+
+struct BState {
+  int level;
+  int phase;
+  int done;
+  int error;
+  int levels;
+  int phases;
+  int pad[2];
+  unsigned long long bu_mask;
+  unsigned long long alg_bytes;
+  unsigned long long total_edges;
+  unsigned long long total_vertices;
+  unsigned long long active[8];
+};
+
+struct BControl {
+  BState st[2];
+  BCounters cnt[3];
+};
+
+struct BArgs {
+  const long long *off;
+  const int *adj;
+  const int *hub_block;
+  const int *piece_v;
+  const int *piece_k;
+  unsigned long long *seen;
+  unsigned long long *lv;
+  unsigned long long *jf[2];
+  int *out;
+  BControl *ctl;
+  long long m;
+  long long nW;  // n * W
+  int n;
+  int nwords;
+  int k;
+  int lcap;
+  int n_hub_block;
+  int n_pieces;
+  int t_wave, t_block, piece;
+  int mode, alpha;
+  int probe;
+  int chunk_words;
+};
+
+template <int W>
+struct alignas(8 * W) Mask {
+  unsigned long long w[W];
+};
+
+template <int W>
+__device__ __forceinline__ Mask<W> load_mask(const unsigned long long *base, long long v) {
+  return *reinterpret_cast<const Mask<W> *>(base + v * W);
+}
+template <int W>
+__device__ __forceinline__ void store_mask(unsigned long long *base, long long v, const Mask<W> &m) {
+  *reinterpret_cast<Mask<W> *>(base + v * W) = m;
+}
+
+__device__ __forceinline__ unsigned long long *level_ptr(const BArgs &a, int L) { return a.lv + (long long)L * a.nW; }
+
+template <int BS>
+struct BSmem {
+  static constexpr int NW = BS / kWave;
+  unsigned long long cand[NW][kWave];
+  int excl[NW][kWave];
+  unsigned char d8[NW][kWave * 68];  // output staging: 64 lanes x 64 sources (+4 pad)
+  unsigned long long c[14];
+  BState S;
+};
+
+__device__ __forceinline__ bool bitv(const unsigned long long *bm, int v) {
+  return ((bm[v >> 6] >> (v & 63)) & 1ull) != 0;
+}
+
+// -- counters ----------------------------------------------------------------
+
+template <int BS>
+__device__ __forceinline__ void block_counts_reset(BSmem<BS> &sm) {
+  if (threadIdx.x < 14) sm.c[threadIdx.x] = 0;
+}
+
+// Reduces per-lane values over the wavefront and adds them to LDS.
+template <int BS, int W>
+__device__ __forceinline__ void wave_counts(BSmem<BS> &sm, unsigned long long jw, unsigned long long mf,
+                                            unsigned long long alg, unsigned long long munf, const Mask<W> &act,
+                                            unsigned long long edges, unsigned long long verts) {
+  unsigned long long vals[6] = {jw, mf, alg, munf, edges, verts};
+  Mask<W> ac = act;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) vals[j] += __shfl_xor(vals[j], o);
+#pragma unroll
+    for (int w = 0; w < W; ++w) ac.w[w] |= __shfl_xor(ac.w[w], o);
+  }
+  if (lane_id() == 0) {
+    const int idx[6] = {C_JWORDS, C_MF, C_ALG, C_MUNF, C_EDGES, C_VERTS};
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+      if (vals[j]) atomicAdd(&sm.c[idx[j]], vals[j]);
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if (ac.w[w]) atomicOr(&sm.c[C_ACTIVE + w], ac.w[w]);
+  }
+}
+
+template <int BS>
+__device__ __forceinline__ void block_counts_flush(const BArgs &a, BSmem<BS> &sm, int slot) {
+  __syncthreads();
+  if (threadIdx.x < 14) {
+    const unsigned long long x = sm.c[threadIdx.x];
+    if (x) {
+      unsigned long long *dst = &a.ctl->cnt[slot].v[(blockIdx.x % kBStripes) * kBStripeU64 + threadIdx.x];
+      if (threadIdx.x >= C_ACTIVE && threadIdx.x < C_ACTIVE + 8) atomicOr(dst, x);
+      else atomicAdd(dst, x);
+    }
+  }
+}
+
+// -- state -------------------------------------------------------------------
+
+__device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned long long *tot, int W) {
+  BState s = p;
+  if (p.done) return s;
+  s.phases = p.phases + 1;
+  if (p.phase == BPH_TD) {
+    s.phase = BPH_APPLY;
+    return s;
+  }
+  if (p.phase == BPH_OUTPUT) {
+    s.total_edges = tot[C_EDGES];
+    s.total_vertices = tot[C_VERTS];
+    s.alg_bytes += 4ull * 64ull * (unsigned long long)W * (unsigned long long)a.n;
+    s.done = 1;
+    return s;
+  }
+  const int L = p.level;  // -1 before level 0
+  if (L >= 0 && p.phase == BPH_BU && L < 64) s.bu_mask |= 1ull << L;
+  s.alg_bytes += tot[C_ALG];
+  if (tot[C_JWORDS] == 0) {
+    s.levels = L + 1;
+    s.phase = BPH_OUTPUT;
+    return s;
+  }
+  if (L + 1 > a.lcap) {
+    s.error |= BERR_LEVELS;
+    s.done = 1;
+    return s;
+  }
+  for (int w = 0; w < 8; ++w) s.active[w] = w < W ? tot[C_ACTIVE + w] : 0ull;
+  const unsigned long long mf = tot[C_MF], munf = tot[C_MUNF];
+  bool bu;
+  switch (a.mode) {
+    case ENT_DIR_TOPDOWN: bu = false; break;
+    case ENT_DIR_BOTTOMUP: bu = true; break;
+    case ENT_DIR_REFERENCE:
+      bu = p.phase == BPH_BU || (L + 1 >= 1 && 10ull * mf >= 3ull * (unsigned long long)a.m);
+      break;
+    default: bu = mf * (unsigned long long)a.alpha > munf;
+  }
+  s.level = L + 1;
+  s.phase = bu ? BPH_BU : BPH_TD;
+  return s;
+}
+
+template <int BS, int W>
+__device__ __forceinline__ void begin_blaunch(const BArgs &a, BSmem<BS> &sm, long long i) {
+  if (threadIdx.x < kWave) {
+    BControl *c = a.ctl;
+    const int lane = threadIdx.x;
+    const BCounters &pc = c->cnt[(i + 2) % 3];
+    unsigned long long t[14];
+#pragma unroll
+    for (int j = 0; j < 14; ++j) t[j] = pc.v[lane * kBStripeU64 + j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+      for (int j = 0; j < 14; ++j) {
+        const unsigned long long x = __shfl_xor(t[j], o);
+        t[j] = (j >= C_ACTIVE && j < C_ACTIVE + 8) ? (t[j] | x) : (t[j] + x);
+      }
+    }
+    if (lane == 0) {
+      const BState S = next_bstate(a, c->st[(i + 1) & 1], t, W);
+      sm.S = S;
+      if (blockIdx.x == 0) c->st[i & 1] = S;
+    }
+    if (blockIdx.x == 0 && lane < 14)
+      for (int s = 0; s < kBStripes; ++s) c->cnt[(i + 1) % 3].v[s * kBStripeU64 + lane] = 0;
+  }
+  block_counts_reset(sm);
+  __syncthreads();
+}
+
+// -- phases ------------------------------------------------------------------
+
+template <int W>
+__device__ __forceinline__ void push_edge(const BArgs &a, unsigned long long *dst, int u, const Mask<W> &f) {
+  const Mask<W> sd = load_mask<W>(a.seen, u);
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const unsigned long long nb = f.w[w] & ~sd.w[w];
+    if (nb) atomicOr(dst + (long long)u * W + w, nb);
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void push_range(const BArgs &a, unsigned long long *dst, long long s, long long e,
+                                           int t, int stride, const Mask<W> &f) {
+  for (long long i = s + t; i < e; i += stride) push_edge<W>(a, dst, a.adj[i], f);
+}
+
+template <int BS, int W>
+__device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
+  const unsigned long long *fs = level_ptr(a, L);
+  unsigned long long *dst = level_ptr(a, L + 1);
+  const unsigned long long *jf = a.jf[L & 1];
+  for (int p = blockIdx.x; p < a.n_pieces; p += gridDim.x) {
+    const int v = a.piece_v[p];
+    if (!bitv(jf, v)) continue;
+    const Mask<W> f = load_mask<W>(fs, v);
+    const long long s = a.off[v] + (long long)a.piece_k[p] * a.piece;
+    push_range<W>(a, dst, s, min(a.off[v + 1], s + a.piece), threadIdx.x, BS, f);
+  }
+  for (int h = blockIdx.x; h < a.n_hub_block; h += gridDim.x) {
+    const int v = a.hub_block[h];
+    if (!bitv(jf, v)) continue;
+    const Mask<W> f = load_mask<W>(fs, v);
+    push_range<W>(a, dst, a.off[v], a.off[v + 1], threadIdx.x, BS, f);
+  }
+  const int wid = threadIdx.x / kWave, lane = lane_id();
+  unsigned long long *wc = sm.cand[wid];
+  int *we = sm.excl[wid];
+  const int CW = a.chunk_words;
+  const long long nchunks = ((long long)a.nwords + CW - 1) / CW;
+  const long long nwv = (long long)gridDim.x * (BS / kWave);
+  for (long long ch = (long long)blockIdx.x * (BS / kWave) + wid; ch < nchunks; ch += nwv) {
+    const long long w = ch * CW + lane;
+    const unsigned long long word = (lane < CW && w < a.nwords) ? jf[w] : 0ull;
+    if (__ballot(word != 0) == 0) continue;
+    const int cnt = __popcll(word);
+    const int incl = wave_incl_scan(cnt);
+    wc[lane] = word;
+    we[lane] = incl - cnt;
+    wave_sync();
+    const int total = wave_read(incl, 63);
+    for (int b0 = 0; b0 < total; b0 += kWave) {
+      const int kk = b0 + lane;
+      int v = 0;
+      long long s = 0, e = 0;
+      if (kk < total) {
+        int lo = 0, hi = kWave - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (we[mid] <= kk) lo = mid;
+          else hi = mid - 1;
+        }
+        v = int((ch * CW + lo) * 64 + select_bit(wc[lo], kk - we[lo]));
+        s = a.off[v];
+        e = a.off[v + 1];
+        if (e - s < a.t_wave) {
+          const Mask<W> f = load_mask<W>(fs, v);
+          push_range<W>(a, dst, s, e, 0, 1, f);
+        }
+      }
+      const long long d = e - s;
+      unsigned long long wm = __ballot(kk < total && d >= a.t_wave && d < a.t_block);
+      while (wm) {
+        const int src = __ffsll((long long)wm) - 1;
+        wm &= wm - 1;
+        const int vv = __shfl(v, src);
+        const long long ss = __shfl(s, src), ee = __shfl(e, src);
+        const Mask<W> f = load_mask<W>(fs, vv);
+        push_range<W>(a, dst, ss, ee, lane, kWave, f);
+      }
+    }
+    wave_sync();
+  }
+}
+
+// Shared tail of APPLY and BU for the vertex of this lane: new bits x (the
+// lane's vertex v < n when valid), seen update, next-level zeroing, counts.
+template <int W>
+__device__ __forceinline__ void settle_vertex(const BArgs &a, int L, bool valid, long long v, const Mask<W> &sd,
+                                              const Mask<W> &x, long long deg, const unsigned long long *active,
+                                              unsigned long long &jw, unsigned long long &mf, unsigned long long &alg,
+                                              unsigned long long &munf, Mask<W> &act, bool &any) {
+  any = false;
+  if (!valid) return;
+  int nzw = 0;
+  Mask<W> ns;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    nzw += x.w[w] != 0;
+    ns.w[w] = sd.w[w] | x.w[w];
+    act.w[w] |= x.w[w];
+  }
+  any = nzw != 0;
+  if (any) {
+    store_mask<W>(a.seen, v, ns);
+    jw += (unsigned long long)nzw;
+    mf += (unsigned long long)deg;
+    alg += (unsigned long long)nzw * (8ull + 12ull * (unsigned long long)deg);
+  }
+  bool unf = false;
+#pragma unroll
+  for (int w = 0; w < W; ++w) unf |= (active[w] & ~ns.w[w]) != 0;
+  if (unf) munf += (unsigned long long)deg;
+  Mask<W> z;
+#pragma unroll
+  for (int w = 0; w < W; ++w) z.w[w] = 0;
+  store_mask<W>(level_ptr(a, L + 2), v, z);
+}
+
+template <int BS, int W>
+__device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
+  unsigned long long *nx = level_ptr(a, L + 1);
+  unsigned long long *jn = a.jf[(L + 1) & 1];
+  const int lane = lane_id();
+  const long long nwv = (long long)gridDim.x * (BS / kWave);
+  unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
+  Mask<W> act;
+#pragma unroll
+  for (int w = 0; w < W; ++w) act.w[w] = 0;
+  for (long long wd = (long long)blockIdx.x * (BS / kWave) + threadIdx.x / kWave; wd < a.nwords; wd += nwv) {
+    const long long v = wd * 64 + lane;
+    const bool valid = v < a.n;
+    Mask<W> x, sd;
+#pragma unroll
+    for (int w = 0; w < W; ++w) x.w[w] = sd.w[w] = 0;
+    long long deg = 0;
+    if (valid) {
+      x = load_mask<W>(nx, v);
+      sd = load_mask<W>(a.seen, v);
+      deg = a.off[v + 1] - a.off[v];
+    }
+    bool any;
+    settle_vertex<W>(a, L, valid, v, sd, x, deg, sm.S.active, jw, mf, alg, munf, act, any);
+    const unsigned long long bits = __ballot(any);
+    if (lane == 0) jn[wd] = bits;
+  }
+  wave_counts<BS, W>(sm, jw, mf, alg, munf, act, 0, 0);
+}
+
+template <int BS, int W>
+__device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
+  const unsigned long long *fs = level_ptr(a, L);
+  unsigned long long *nx = level_ptr(a, L + 1);
+  unsigned long long *jn = a.jf[(L + 1) & 1];
+  const int lane = lane_id();
+  const long long nwv = (long long)gridDim.x * (BS / kWave);
+  unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
+  Mask<W> act;
+#pragma unroll
+  for (int w = 0; w < W; ++w) act.w[w] = 0;
+  for (long long wd = (long long)blockIdx.x * (BS / kWave) + threadIdx.x / kWave; wd < a.nwords; wd += nwv) {
+    const long long v = wd * 64 + lane;
+    const bool valid = v < a.n;
+    Mask<W> sd, tgt, acc;
+    bool need = false;
+    long long s = 0, e = 0, p = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) sd.w[w] = tgt.w[w] = acc.w[w] = 0;
+    if (valid) {
+      sd = load_mask<W>(a.seen, v);
+      s = a.off[v];
+      e = a.off[v + 1];
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        tgt.w[w] = sm.S.active[w] & ~sd.w[w];
+        need |= tgt.w[w] != 0;
+      }
+      need = need && e > s;
+    }
+    bool complete = !need;
+    if (need) {
+      const long long pe = min(e, s + a.probe);
+      for (p = s; p < pe; ++p) {
+        const Mask<W> f = load_mask<W>(fs, a.adj[p]);
+        bool all = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          acc.w[w] |= f.w[w];
+          all &= (acc.w[w] & tgt.w[w]) == tgt.w[w];
+        }
+        if (all) {
+          complete = true;
+          break;
+        }
+      }
+    }
+    // Long lists: the whole wavefront scans one vertex at a time, OR-reducing
+    // the masks each step and stopping once the targets are covered.
+    unsigned long long lmask = __ballot(need && !complete && p < e);
+    while (lmask) {
+      const int src = __ffsll((long long)lmask) - 1;
+      lmask &= lmask - 1;
+      const long long pp = __shfl(p, src), ee = __shfl(e, src);
+      Mask<W> t2, a2;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        t2.w[w] = __shfl(tgt.w[w], src);
+        a2.w[w] = __shfl(acc.w[w], src);
+      }
+      for (long long q0 = pp; q0 < ee; q0 += kWave) {
+        const long long q = q0 + lane;
+        Mask<W> f;
+#pragma unroll
+        for (int w = 0; w < W; ++w) f.w[w] = 0;
+        if (q < ee) f = load_mask<W>(fs, a.adj[q]);
+        bool all = true;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          unsigned long long x = f.w[w];
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
+          a2.w[w] |= x;
+          all &= (a2.w[w] & t2.w[w]) == t2.w[w];
+        }
+        if (all) break;
+      }
+      if (lane == src) acc = a2;
+    }
+    Mask<W> x;
+#pragma unroll
+    for (int w = 0; w < W; ++w) x.w[w] = acc.w[w] & ~sd.w[w];
+    if (valid) store_mask<W>(nx, v, x);
+    bool any;
+    settle_vertex<W>(a, L, valid, v, sd, x, e - s, sm.S.active, jw, mf, alg, munf, act, any);
+    const unsigned long long bits = __ballot(any);
+    if (lane == 0) jn[wd] = bits;
+  }
+  wave_counts<BS, W>(sm, jw, mf, alg, munf, act, 0, 0);
+}
+
+// Output: per 64-vertex segment, replay the level masks of each 64-source
+// group into an LDS table of levels (u8, 0xFF = unreached) and store the 64
+// rows of the group with 256-byte coalesced writes.
+template <int BS, int W>
+__device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
+  const int wid = threadIdx.x / kWave, lane = lane_id();
+  unsigned char *d8 = sm.d8[wid];
+  const long long nwv = (long long)gridDim.x * (BS / kWave);
+  unsigned long long edges = 0, verts = 0;
+  for (long long wd = (long long)blockIdx.x * (BS / kWave) + wid; wd < a.nwords; wd += nwv) {
+    const long long v = wd * 64 + lane;
+    const bool valid = v < a.n;
+    if (valid) {
+      const Mask<W> sd = load_mask<W>(a.seen, v);
+      int pc = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) pc += __popcll(sd.w[w]);
+      verts += (unsigned long long)pc;
+      edges += (unsigned long long)pc * (unsigned long long)(a.off[v + 1] - a.off[v]);
+    }
+    for (int g = 0; g < W; ++g) {
+      unsigned int *row = reinterpret_cast<unsigned int *>(d8 + lane * 68);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) row[j] = 0xFFFFFFFFu;
+      if (valid) {
+        for (int L = 0; L < levels; ++L) {
+          unsigned long long mk = level_ptr(a, L)[v * W + g];
+          while (mk) {
+            const int b = __ffsll((long long)mk) - 1;
+            mk &= mk - 1;
+            d8[lane * 68 + b] = (unsigned char)L;
+          }
+        }
+      }
+      wave_sync();
+      if (valid) {
+        const int nb = min(64, a.k - 64 * g);
+        for (int b = 0; b < nb; ++b) {
+          const unsigned char d = d8[lane * 68 + b];
+          a.out[(long long)(64 * g + b) * a.n + v] = d == 0xFF ? -1 : int(d);
+        }
+      }
+      wave_sync();
+    }
+  }
+  Mask<W> none;
+#pragma unroll
+  for (int w = 0; w < W; ++w) none.w[w] = 0;
+  wave_counts<BS, W>(sm, 0, 0, 0, 0, none, edges, verts);
+}
+
+template <int BS, int W>
+__global__ __launch_bounds__(BS) void k_blevel(BArgs a, long long i) {
+  __shared__ BSmem<BS> sm;
+  begin_blaunch<BS, W>(a, sm, i);
+  const BState &S = sm.S;
+  if (S.done) return;
+  const int slot = int(i % 3);
+  switch (S.phase) {
+    case BPH_TD: btd_phase<BS, W>(a, sm, S.level); break;
+    case BPH_APPLY: bapply_phase<BS, W>(a, sm, S.level); break;
+    case BPH_BU: bbu_phase<BS, W>(a, sm, S.level); break;
+    default: boutput_phase<BS, W>(a, sm, S.levels); break;
+  }
+  if (S.phase != BPH_TD) block_counts_flush(a, sm, slot);
+}
+
+// Zeroes seen, lv[0], lv[1] and the joint frontier; clears counters; sets the
+// pseudo-level -1 state.
+__global__ void k_binit_zero(BArgs a) {
+  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nt = (long long)gridDim.x * blockDim.x;
+  unsigned long long *l0 = a.lv, *l1 = a.lv + a.nW;
+  for (long long i = gt; i < a.nW; i += nt) {
+    a.seen[i] = 0;
+    l0[i] = 0;
+    l1[i] = 0;
+  }
+  for (long long w = gt; w < a.nwords; w += nt) {
+    a.jf[0][w] = 0;
+    a.jf[1][w] = 0;
+  }
+  constexpr long long kC = (long long)kBStripes * kBStripeU64;
+  for (long long i = gt; i < 3 * kC; i += nt) a.ctl->cnt[i / kC].v[i % kC] = 0;
+  if (gt == 0) {
+    BState S{};
+    S.level = -1;
+    S.phase = BPH_APPLY;
+    a.ctl->st[0] = S;
+    a.ctl->st[1] = S;
+  }
+}
+
+// Sets the source bits (level 0) and the level-0 counts computed on the host.
+__global__ void k_binit_src(BArgs a, const int *roots, unsigned long long jw, unsigned long long mf,
+                            unsigned long long alg, unsigned long long munf) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < a.k) {
+    const int v = roots[s];
+    const int W = int(a.nW / a.n);
+    const unsigned long long bit = 1ull << (s & 63);
+    atomicOr(&a.lv[(long long)v * W + (s >> 6)], bit);
+    atomicOr(&a.seen[(long long)v * W + (s >> 6)], bit);
+    atomicOr(&a.jf[0][v >> 6], 1ull << (v & 63));
+    atomicOr(&a.ctl->cnt[2].v[C_ACTIVE + (s >> 6)], bit);
+  }
+  if (s == 0) {
+    unsigned long long *c = a.ctl->cnt[2].v;
+    c[C_JWORDS] = jw;
+    c[C_MF] = mf;
+    c[C_ALG] = alg;
+    c[C_MUNF] = munf;
+  }
+}
+
+// Position-keyed checksum of each row (sum over i of mix(i, value)).
+__global__ void k_row_hash(const int *dev, long long cols, unsigned long long *out) {
+  const long long r = blockIdx.x;
+  unsigned long long h = 0;
+  for (long long i = threadIdx.x; i < cols; i += blockDim.x) {
+    unsigned long long z = ((unsigned long long)i << 32) ^ (unsigned int)dev[r * cols + i];
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    h += z ^ (z >> 31);
+  }
+  __shared__ unsigned long long red[256];
+  red[threadIdx.x] = h;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[r] = red[0];
+}
+
+struct BatchWorkspace {
+  int words = 0;      // W (1, 2, 4 or 8)
+  int lcap = 0;       // max joint levels
+  int64_t n = 0;
+  unsigned long long *seen = nullptr;
+  unsigned long long *lv = nullptr;
+  unsigned long long *jf[2] = {nullptr, nullptr};
+  int *roots = nullptr;
+  BControl *ctl = nullptr;
+  BControl *h_ctl = nullptr;
+  size_t bytes = 0;
+  int expect_launches = 16;
+};
+
+}  // namespace ent
+
+using namespace ent;
+
+namespace {
+
+constexpr int kBatchBS = 256;
+constexpr int kDefaultLevels = 32;
+
+int words_for(int k) { return k <= 64 ? 1 : k <= 128 ? 2 : k <= 256 ? 4 : 8; }
+
+size_t ws_bytes(int64_t n, int W, int lcap) {
+  const size_t nW = size_t(n) * size_t(W);
+  const size_t nwords = size_t((n + 63) / 64);
+  return 8 * nW + 8 * nW * size_t(lcap + 3) + 16 * nwords + sizeof(BControl) + 4 * ENT_BATCH_MAX_SOURCES;
+}
+
+int reserve(ent_graph *g, int k, int lcap) {
+  if (k < 1 || k > ENT_BATCH_MAX_SOURCES) return set_error(ENT_EINVAL, "k=%d outside [1, 512]", k);
+  if (lcap < 1 || lcap > 250) return set_error(ENT_EINVAL, "max_levels=%d outside [1, 250]", lcap);
+  const int W = words_for(k);
+  BatchWorkspace *b = g->batch;
+  if (b && b->words >= W && b->lcap >= lcap) return ENT_OK;
+  if (b) {
+    batch_release(g);
+    b = nullptr;
+  }
+  b = new (std::nothrow) BatchWorkspace();
+  if (!b) return set_error(ENT_ENOMEM, "out of host memory");
+  b->words = W;
+  b->lcap = lcap;
+  b->n = g->n;
+  const size_t nW = size_t(g->n) * size_t(W);
+  int rc;
+  g->batch = b;
+  if ((rc = dev_alloc(g, reinterpret_cast<void **>(&b->seen), 8 * nW)) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&b->lv), 8 * nW * size_t(lcap + 3))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&b->jf[0]), 8 * size_t(g->nwords))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&b->jf[1]), 8 * size_t(g->nwords))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&b->roots), 4 * ENT_BATCH_MAX_SOURCES)) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&b->ctl), sizeof(BControl)))) {
+    batch_release(g);
+    return rc;
+  }
+  if (hipHostMalloc(reinterpret_cast<void **>(&b->h_ctl), sizeof(BControl), hipHostMallocDefault) != hipSuccess) {
+    batch_release(g);
+    return set_error(ENT_ENOMEM, "pinned host memory");
+  }
+  b->bytes = ws_bytes(g->n, W, lcap);
+  return ENT_OK;
+}
+
+template <int W>
+const void *blevel_kernel() {
+  return reinterpret_cast<const void *>(&k_blevel<kBatchBS, W>);
+}
+
+const void *blevel_for(int W) {
+  switch (W) {
+    case 1: return blevel_kernel<1>();
+    case 2: return blevel_kernel<2>();
+    case 4: return blevel_kernel<4>();
+    default: return blevel_kernel<8>();
+  }
+}
+
+int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) {
+  BatchWorkspace *b = g->batch;
+  const int W = b->words;
+  BArgs a{};
+  a.off = g->off;
+  a.adj = g->adj;
+  a.hub_block = g->hub_block;
+  a.piece_v = g->piece_v;
+  a.piece_k = g->piece_k;
+  a.seen = b->seen;
+  a.lv = b->lv;
+  a.jf[0] = b->jf[0];
+  a.jf[1] = b->jf[1];
+  a.out = out;
+  a.ctl = b->ctl;
+  a.m = g->m;
+  a.nW = g->n * W;
+  a.n = int(g->n);
+  a.nwords = int(g->nwords);
+  a.k = k;
+  a.lcap = b->lcap;
+  a.n_hub_block = g->n_hub_block;
+  a.n_pieces = g->n_pieces;
+  const ent_bfs_config &c = g->cfg;
+  a.t_wave = c.t_wave;
+  a.t_block = c.t_block;
+  a.piece = c.piece;
+  a.mode = c.direction;
+  a.alpha = c.batch_alpha > 0 ? c.batch_alpha : 4;
+  a.probe = c.bu_probe;
+  a.chunk_words = c.bu_chunk_words;
+  // Level-0 counts on the host: distinct (vertex, word) pairs of the roots.
+  std::vector<std::pair<int32_t, int>> vw;
+  std::vector<int32_t> vs;
+  for (int s = 0; s < k; ++s) {
+    vw.push_back({roots[s], s >> 6});
+    vs.push_back(roots[s]);
+  }
+  std::sort(vw.begin(), vw.end());
+  vw.erase(std::unique(vw.begin(), vw.end()), vw.end());
+  std::sort(vs.begin(), vs.end());
+  vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
+  unsigned long long mf = 0, alg = 0;
+  for (int32_t v : vs) mf += (unsigned long long)g->h_deg[size_t(v)];
+  for (auto &p : vw) alg += 8ull + 12ull * (unsigned long long)g->h_deg[size_t(p.first)];
+  unsigned long long munf = (unsigned long long)g->m;  // every vertex still needs sources
+  HIP_TRY(hipMemcpyAsync(b->roots, roots, sizeof(int32_t) * size_t(k), hipMemcpyHostToDevice, g->stream));
+  const int grid = g->cus * 8;
+  {
+    void *args[] = {&a};
+    HIP_TRY(hipLaunchKernel(reinterpret_cast<const void *>(&k_binit_zero), dim3(grid), dim3(256), args, 0, g->stream));
+  }
+  {
+    unsigned long long jw = vw.size();
+    void *args[] = {&a, &b->roots, &jw, &mf, &alg, &munf};
+    HIP_TRY(hipLaunchKernel(reinterpret_cast<const void *>(&k_binit_src), dim3((k + 255) / 256), dim3(256), args, 0,
+                            g->stream));
+  }
+  const void *kl = blevel_for(W);
+  const int lgrid = g->cus * std::max(1, c.blocks_per_cu > 0 ? c.blocks_per_cu : 8);
+  long long i = 0;
+  int kk = std::max(2, b->expect_launches);
+  const long long max_launches = 4LL * (b->lcap + 4);
+  for (;;) {
+    for (int j = 0; j < kk; ++j, ++i) {
+      void *args[] = {&a, &i};
+      HIP_TRY(hipLaunchKernel(kl, dim3(lgrid), dim3(kBatchBS), args, 0, g->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(b->h_ctl, b->ctl, sizeof(BState) * 2, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    *fin = b->h_ctl->st[(i - 1) & 1];
+    if (fin->done) break;
+    if (i > max_launches) return set_error(ENT_EINTERNAL, "batch did not finish in %lld launches", i);
+    kk = 2;
+  }
+  b->expect_launches = std::min(fin->phases + 1, 512);
+  g->last_grid = lgrid;
+  return ENT_OK;
+}
+
+}  // namespace
+
+namespace ent {
+void batch_release(ent_graph *g) {
+  BatchWorkspace *b = g ? g->batch : nullptr;
+  if (!b) return;
+  const size_t nW = size_t(b->n) * size_t(b->words);
+  dev_free(g, b->seen, 8 * nW);
+  dev_free(g, b->lv, 8 * nW * size_t(b->lcap + 3));
+  dev_free(g, b->jf[0], 8 * size_t(g->nwords));
+  dev_free(g, b->jf[1], 8 * size_t(g->nwords));
+  dev_free(g, b->roots, 4 * ENT_BATCH_MAX_SOURCES);
+  dev_free(g, b->ctl, sizeof(BControl));
+  if (b->h_ctl) ignore(hipHostFree(b->h_ctl));
+  delete b;
+  g->batch = nullptr;
+}
+}  // namespace ent
+
+extern "C" {
+
+int64_t ent_batch_workspace_bytes(const ent_graph *g, int32_t k, int32_t max_levels) {
+  if (!g || k < 1 || k > ENT_BATCH_MAX_SOURCES) return -1;
+  return int64_t(ws_bytes(g->n, words_for(k), max_levels > 0 ? max_levels : kDefaultLevels));
+}
+
+int ent_batch_reserve(ent_graph *g, int32_t k, int32_t max_levels) {
+  clear_error();
+  if (!g) return set_error(ENT_EINVAL, "NULL graph");
+  std::lock_guard<std::mutex> lk(g->mu);
+  HIP_TRY(hipSetDevice(g->device));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  const int lcap = max_levels > 0 ? max_levels : (g->batch ? g->batch->lcap : kDefaultLevels);
+  return reserve(g, k, lcap);
+}
+
+int ent_batch_release(ent_graph *g) {
+  clear_error();
+  if (!g) return set_error(ENT_EINVAL, "NULL graph");
+  std::lock_guard<std::mutex> lk(g->mu);
+  HIP_TRY(hipSetDevice(g->device));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  batch_release(g);
+  return ENT_OK;
+}
+
+int ent_bfs_batch_device(ent_graph *g, const int32_t *roots, int32_t k, int32_t *depths_dev,
+                         ent_batch_stats *st) {
+  clear_error();
+  if (!g || !roots || !depths_dev) return set_error(ENT_EINVAL, "NULL argument");
+  if (k < 1 || k > ENT_BATCH_MAX_SOURCES) return set_error(ENT_EINVAL, "k=%d outside [1, 512]", k);
+  for (int s = 0; s < k; ++s)
+    if (roots[s] < 0 || roots[s] >= g->n) return set_error(ENT_EINVAL, "root %d = %d out of range", s, roots[s]);
+  std::lock_guard<std::mutex> lk(g->mu);
+  const auto t0 = std::chrono::steady_clock::now();
+  HIP_TRY(hipSetDevice(g->device));
+  int rc = reserve(g, k, g->batch ? g->batch->lcap : kDefaultLevels);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(g->ev0, g->stream));
+  BState fin{};
+  rc = run_batch(g, roots, k, depths_dev, &fin);
+  if (rc) {
+    ignore(hipStreamSynchronize(g->stream));
+    return rc;
+  }
+  HIP_TRY(hipEventRecord(g->ev1, g->stream));
+  HIP_TRY(hipEventSynchronize(g->ev1));
+  const auto t1 = std::chrono::steady_clock::now();
+  if (fin.error & BERR_LEVELS)
+    return set_error(ENT_ENOMEM, "batch needs more than %d joint levels; reserve more with ent_batch_reserve",
+                     g->batch->lcap);
+  if (fin.error) return set_error(ENT_EINTERNAL, "device reported error flags 0x%x", fin.error);
+  if (st) {
+    std::memset(st, 0, sizeof *st);
+    st->k = k;
+    st->words = g->batch->words;
+    st->levels = fin.levels;
+    st->phases = fin.phases;
+    st->reached_edges = int64_t(fin.total_edges);
+    st->reached_vertices = int64_t(fin.total_vertices);
+    st->alg_bytes = int64_t(fin.alg_bytes);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g->ev0, g->ev1) == hipSuccess) st->device_ms = ms;
+    st->host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    st->bu_level_mask = fin.bu_mask;
+  }
+  return ENT_OK;
+}
+
+int ent_bfs_batch(ent_graph *g, const int32_t *roots, int32_t k, int32_t *depths_host, ent_batch_stats *st) {
+  clear_error();
+  if (!g || !depths_host) return set_error(ENT_EINVAL, "NULL argument");
+  if (k < 1 || k > ENT_BATCH_MAX_SOURCES) return set_error(ENT_EINVAL, "k=%d outside [1, 512]", k);
+  void *dev = nullptr;
+  int rc = ent_device_alloc(g, int64_t(k) * g->n * 4, &dev);
+  if (rc) return rc;
+  rc = ent_bfs_batch_device(g, roots, k, static_cast<int32_t *>(dev), st);
+  if (!rc) rc = ent_copy_to_host(g, static_cast<int32_t *>(dev), depths_host, int64_t(k) * g->n);
+  ent_device_free(g, dev);
+  return rc;
+}
+
+int ent_device_alloc(ent_graph *g, int64_t bytes, void **out) {
+  clear_error();
+  if (!g || !out || bytes < 0) return set_error(ENT_EINVAL, "bad argument");
+  HIP_TRY(hipSetDevice(g->device));
+  *out = nullptr;
+  hipError_t e = hipMalloc(out, size_t(std::max<int64_t>(bytes, 8)));
+  if (e != hipSuccess) {
+    ignore(hipGetLastError());
+    return set_error(ENT_ENOMEM, "hipMalloc(%lld) failed: %s", (long long)bytes, hipGetErrorString(e));
+  }
+  return ENT_OK;
+}
+
+int ent_device_free(ent_graph *g, void *p) {
+  clear_error();
+  if (!g) return set_error(ENT_EINVAL, "NULL graph");
+  if (!p) return ENT_OK;
+  HIP_TRY(hipSetDevice(g->device));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  HIP_TRY(hipFree(p));
+  return ENT_OK;
+}
+
+int ent_device_row_hash(ent_graph *g, const int32_t *dev, int64_t rows, int64_t cols, uint64_t *hashes_host) {
+  clear_error();
+  if (!g || !dev || !hashes_host || rows < 1 || cols < 1 || rows > (1 << 20))
+    return set_error(ENT_EINVAL, "bad argument");
+  std::lock_guard<std::mutex> lk(g->mu);
+  HIP_TRY(hipSetDevice(g->device));
+  unsigned long long *d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(unsigned long long) * size_t(rows)));
+  void *args[] = {const_cast<int32_t **>(&dev), &cols, &d};
+  hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(&k_row_hash), dim3(unsigned(rows)), dim3(256), args, 0,
+                                 g->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(hashes_host, d, sizeof(unsigned long long) * size_t(rows), hipMemcpyDeviceToHost, g->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  ignore(hipFree(d));
+  if (e != hipSuccess) return set_error(ENT_EDEVICE, "row hash: %s", hipGetErrorString(e));
+  return ENT_OK;
+}
+
+}  // extern "C"

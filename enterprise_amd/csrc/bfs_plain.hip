@@ -192,6 +192,22 @@ __device__ __forceinline__ void td_mark(const Args &a, int u) {
   if (!bit_of(a.visited, u)) a.mark[u] = 1;
 }
 
+// Marks the unvisited neighbours adj[i] for i = s + t, s + t + stride, ...
+// < e; four independent edges per step keep four loads in flight.
+__device__ __forceinline__ void td_mark_range(const Args &a, long long s, long long e, int t, int stride) {
+  long long i = s + t;
+  for (; i + 3LL * stride < e; i += 4LL * stride) {
+    const int u0 = a.adj[i], u1 = a.adj[i + stride], u2 = a.adj[i + 2 * stride], u3 = a.adj[i + 3 * stride];
+    const bool m0 = !bit_of(a.visited, u0), m1 = !bit_of(a.visited, u1);
+    const bool m2 = !bit_of(a.visited, u2), m3 = !bit_of(a.visited, u3);
+    if (m0) a.mark[u0] = 1;
+    if (m1) a.mark[u1] = 1;
+    if (m2) a.mark[u2] = 1;
+    if (m3) a.mark[u3] = 1;
+  }
+  for (; i < e; i += stride) td_mark(a, a.adj[i]);
+}
+
 // Top-down level L. Grid-class pieces and block-class hubs come from the
 // static lists (skipped unless in frontier L); thread- and wavefront-class
 // frontier vertices come from a compacting sweep of front[L & 1]. Every
@@ -204,13 +220,12 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
     if (!bit_of(fr, v)) continue;
     const long long s = a.off[v] + (long long)a.piece_k[p] * a.piece;
     const long long e = min(a.off[v + 1], s + a.piece);
-    for (long long i = s + threadIdx.x; i < e; i += BS) td_mark(a, a.adj[i]);
+    td_mark_range(a, s, e, threadIdx.x, BS);
   }
   for (int h = blockIdx.x; h < a.n_hub_block; h += gridDim.x) {
     const int v = a.hub_block[h];
     if (!bit_of(fr, v)) continue;
-    const long long s = a.off[v], e = a.off[v + 1];
-    for (long long i = s + threadIdx.x; i < e; i += BS) td_mark(a, a.adj[i]);
+    td_mark_range(a, a.off[v], a.off[v + 1], threadIdx.x, BS);
   }
   const int wid = threadIdx.x / kWave, lane = lane_id();
   unsigned long long *wc = sm.cand[wid];
@@ -232,9 +247,7 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
         const int v = int((ch * CW + j) * 64 + bp);
         s = a.off[v];
         e = a.off[v + 1];
-        if (e - s < a.t_wave) {
-          for (long long i = s; i < e; ++i) td_mark(a, a.adj[i]);
-        }
+        if (e - s < a.t_wave) td_mark_range(a, s, e, 0, 1);
       }
       const long long d = e - s;
       unsigned long long wm = __ballot(k < total && d >= a.t_wave && d < a.t_block);
@@ -242,7 +255,7 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
         const int src = __ffsll((long long)wm) - 1;
         wm &= wm - 1;
         const long long ss = __shfl(s, src), ee = __shfl(e, src);
-        for (long long i = ss + lane; i < ee; i += kWave) td_mark(a, a.adj[i]);
+        td_mark_range(a, ss, ee, lane, kWave);
       }
     }
     wave_sync();
@@ -321,10 +334,22 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         s = a.off[v];
         e = a.off[v + 1];
         const long long pe = min(e, s + a.bu_probe);
-        for (p = s; p < pe; ++p) {
-          if (bit_of(fr, a.adj[p])) {
+        // Four neighbours per step: their loads overlap; the step stops the
+        // scan if any of them is in the frontier.
+        for (p = s; p + 3 < pe; p += 4) {
+          const int u0 = a.adj[p], u1 = a.adj[p + 1], u2 = a.adj[p + 2], u3 = a.adj[p + 3];
+          // Non-short-circuit OR: all four loads are issued before the test.
+          if ((int(bit_of(fr, u0)) | int(bit_of(fr, u1)) | int(bit_of(fr, u2)) | int(bit_of(fr, u3))) != 0) {
             found = true;
             break;
+          }
+        }
+        if (!found) {
+          for (; p < pe; ++p) {
+            if (bit_of(fr, a.adj[p])) {
+              found = true;
+              break;
+            }
           }
         }
       }
@@ -345,9 +370,9 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         const long long ee = __shfl(e, src);
         const unsigned long long bb = __shfl(bitm, src);
         bool hit_any = false;
-        for (long long q0 = pp; q0 < ee; q0 += kWave) {
-          const long long q = q0 + lane;
-          const bool hit = q < ee && bit_of(fr, a.adj[q]);
+        for (long long q0 = pp; q0 < ee; q0 += 2 * kWave) {
+          const long long q = q0 + lane, q2 = q + kWave;
+          const bool hit = (int(q < ee && bit_of(fr, a.adj[q])) | int(q2 < ee && bit_of(fr, a.adj[q2]))) != 0;
           if (__ballot(hit)) {
             hit_any = true;
             break;
@@ -597,73 +622,17 @@ __global__ void k_selftest(int *out) {
 
 // ===========================================================================
 // Host side.
+#include "ent_internal.h"
+
 using namespace ent;
-
-struct ent_graph {
-  int device = 0;
-  int64_t n = 0, m = 0, n_active = 0, nwords = 0;
-  hipStream_t stream = nullptr;
-  long long *off = nullptr;
-  int *adj = nullptr;
-  unsigned long long *nzmask = nullptr;
-  int *hub_block = nullptr;
-  int *piece_v = nullptr;
-  int *piece_k = nullptr;
-  int n_hub_block = 0, n_pieces = 0;
-  int lists_t_block = -1, lists_t_grid = -1, lists_piece = -1;
-  std::vector<int32_t> h_deg;  // host copy of degrees (for the static lists)
-  int *depth = nullptr;
-  unsigned long long *visited = nullptr;
-  unsigned long long *front[2] = {nullptr, nullptr};
-  unsigned char *mark = nullptr;
-  Control *ctl = nullptr;
-  Control *h_ctl = nullptr;
-  ent_bfs_config cfg{};
-  int cus = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  Args graph_args{};
-  int graph_bs = 0, graph_grid = 0, graph_k = 0;
-  size_t dev_bytes = 0;
-  int last_grid = 0;
-  int expect_launches = 12;
-  std::mutex mu;
-};
-
-#define HIP_TRY(expr)                                                                     \
-  do {                                                                                    \
-    hipError_t e_ = (expr);                                                               \
-    if (e_ != hipSuccess)                                                                 \
-      return set_error(ENT_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));       \
-  } while (0)
 
 namespace {
 
-template <class T>
-inline void ignore(T) {}
-
-int dev_alloc(ent_graph *g, void **p, size_t bytes) {
-  if (bytes == 0) bytes = 8;
-  hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    ignore(hipGetLastError());
-    return set_error(ENT_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-  }
-  g->dev_bytes += bytes;
-  return ENT_OK;
-}
-
-void dev_free(ent_graph *g, void *p, size_t bytes) {
-  if (!p) return;
-  ignore(hipFree(p));
-  g->dev_bytes -= std::min(g->dev_bytes, bytes);
-}
 
 void release(ent_graph *g) {
   if (!g) return;
   ignore(hipSetDevice(g->device));
+  batch_release(g);
   if (g->gexec) ignore(hipGraphExecDestroy(g->gexec));
   if (g->graph) ignore(hipGraphDestroy(g->graph));
   ignore(hipFree(g->off));
@@ -776,6 +745,8 @@ Args make_args(ent_graph *g, int *depth) {
   a.chunk_words = c.bu_chunk_words;
   return a;
 }
+
+static_assert(sizeof(Args) <= sizeof(((ent_graph *)nullptr)->graph_args), "graph_args too small");
 
 struct KernelSet {
   const void *init, *level, *persist;
@@ -903,7 +874,7 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
       const int k = std::max(6, (cfg.graph_levels + 5) / 6 * 6);
       const int grid = level_grid(g);
       if (!g->gexec || g->graph_bs != bs || g->graph_grid != grid || g->graph_k != k ||
-          std::memcmp(&g->graph_args, &a, sizeof(Args)) != 0) {
+          std::memcmp(g->graph_args, &a, sizeof(Args)) != 0) {
         if (g->gexec) ignore(hipGraphExecDestroy(g->gexec));
         if (g->graph) ignore(hipGraphDestroy(g->graph));
         g->gexec = nullptr;
@@ -916,7 +887,7 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
           return set_error(ENT_EDEVICE, "graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
         g->graph = gr;
         HIP_TRY(hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0));
-        g->graph_args = a;
+        std::memcpy(g->graph_args, &a, sizeof(Args));
         g->graph_bs = bs;
         g->graph_grid = grid;
         g->graph_k = k;
@@ -947,15 +918,16 @@ void ent_bfs_default_config(ent_bfs_config *c) {
   c->alpha = 15;
   c->beta = 18;
   c->t_wave = 32;
-  c->t_block = 2048;
-  c->t_grid = 32768;
-  c->piece = 8192;
+  c->t_block = 1024;
+  c->t_grid = 4096;
+  c->piece = 2048;
   c->block_size = 256;
   c->blocks_per_cu = 0;
   c->bu_probe = 16;
   c->bu_chunk_words = 8;
   c->graph_levels = 8;
   c->hub_cache = 0;
+  c->batch_alpha = 4;
 }
 
 int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int64_t m, int device,

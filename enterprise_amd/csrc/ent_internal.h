@@ -1,0 +1,87 @@
+// Host-side definition of the opaque graph handle, shared by the
+// single-source (bfs_plain.hip) and batched (bfs_batch.hip) translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "ent_bfs.h"
+#include "ent_common.h"
+
+namespace ent {
+struct Args;
+struct Control;
+struct BatchWorkspace;
+template <class T>
+inline void ignore(T) {}
+}  // namespace ent
+
+struct ent_graph {
+  int device = 0;
+  int64_t n = 0, m = 0, n_active = 0, nwords = 0;
+  hipStream_t stream = nullptr;
+  long long *off = nullptr;
+  int *adj = nullptr;
+  unsigned long long *nzmask = nullptr;
+  int *hub_block = nullptr;
+  int *piece_v = nullptr;
+  int *piece_k = nullptr;
+  int n_hub_block = 0, n_pieces = 0;
+  int lists_t_block = -1, lists_t_grid = -1, lists_piece = -1;
+  std::vector<int32_t> h_deg;  // host copy of degrees (for the static lists)
+  int *depth = nullptr;
+  unsigned long long *visited = nullptr;
+  unsigned long long *front[2] = {nullptr, nullptr};
+  unsigned char *mark = nullptr;
+  ent::Control *ctl = nullptr;
+  ent::Control *h_ctl = nullptr;
+  ent_bfs_config cfg{};
+  int cus = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  unsigned char graph_args[512] = {};  // launch arguments baked into the captured graph
+  int graph_bs = 0, graph_grid = 0, graph_k = 0;
+  size_t dev_bytes = 0;
+  int last_grid = 0;
+  int expect_launches = 12;
+  ent::BatchWorkspace *batch = nullptr;  // lazily allocated multi-source workspace
+  std::mutex mu;
+};
+
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return ent::set_error(ENT_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));  \
+  } while (0)
+
+namespace ent {
+
+inline int dev_alloc(ent_graph *g, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    ignore(hipGetLastError());
+    return set_error(ENT_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  g->dev_bytes += bytes;
+  return ENT_OK;
+}
+
+inline void dev_free(ent_graph *g, void *p, size_t bytes) {
+  if (!p) return;
+  ignore(hipFree(p));
+  g->dev_bytes -= std::min(g->dev_bytes, bytes);
+}
+
+// Frees the batched workspace (defined in bfs_batch.hip).
+void batch_release(ent_graph *g);
+
+}  // namespace ent

@@ -1,0 +1,81 @@
+/*
+ * ent_batch.h - multi-source BFS on a resident graph (no counterpart in the
+ * reference, which runs one root at a time, bfs_gpu_opt.cuh:370; the form
+ * follows the iBFS idea the reference cites at readme.md:79).
+ *
+ * Up to 512 roots are searched together: each vertex carries W = ceil(k/64)
+ * u64 masks (bit s = "source s"), so one adjacency read serves every source
+ * whose frontier touches it. The result is k int32 depth rows, row-major
+ * (depths[s * n + v]), each identical to what ent_bfs gives for roots[s].
+ *
+ * Memory: besides the graph, a workspace of (max_levels + 3) * n * W * 8
+ * bytes holds the per-level frontier masks (levels are replayed at the end
+ * to write every output row in full, coalesced lines). A batch whose search
+ * needs more levels than reserved fails with ENT_ENOMEM and leaves the handle
+ * usable; reserve more levels and retry.
+ */
+#ifndef ENT_BATCH_H
+#define ENT_BATCH_H
+
+#include <stdint.h>
+
+#include "ent_bfs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ENT_BATCH_MAX_SOURCES 512
+
+typedef struct ent_batch_stats {
+  int32_t k;                /* sources */
+  int32_t words;            /* u64 masks per vertex */
+  int32_t levels;           /* joint levels (max over sources of depth + 1) */
+  int32_t phases;           /* kernel phases run */
+  int64_t reached_edges;    /* sum over sources of M_c(s) */
+  int64_t reached_vertices; /* sum over sources of N_c(s) */
+  int64_t alg_bytes;        /* per 64-source group g: sum over levels l and
+                               v in J_l(g) of (8 + 12 deg v), plus 4*64*n;
+                               summed over groups */
+  double device_ms;         /* HIP-event time of the batch */
+  double host_ms;           /* wall time of the call */
+  uint64_t bu_level_mask;   /* bit l: joint level l ran bottom-up */
+} ent_batch_stats;
+
+/* Reserves the workspace for up to k sources and max_levels joint levels
+ * (0 = keep the current reservation or the default of 32 levels). */
+int ent_batch_reserve(ent_graph *g, int32_t k, int32_t max_levels);
+
+/* Frees the multi-source workspace (the graph stays resident). */
+int ent_batch_release(ent_graph *g);
+
+/* Device bytes the workspace would take for k sources and max_levels. */
+int64_t ent_batch_workspace_bytes(const ent_graph *g, int32_t k,
+                                  int32_t max_levels);
+
+/* Multi-source BFS from roots[0..k-1] (1 <= k <= 512, roots may repeat),
+ * writing k * n int32 depths to device memory depths_dev. */
+int ent_bfs_batch_device(ent_graph *g, const int32_t *roots, int32_t k,
+                         int32_t *depths_dev, ent_batch_stats *stats);
+
+/* Same, copying the k * n depths to host memory depths_host afterwards
+ * (outside the timed search). */
+int ent_bfs_batch(ent_graph *g, const int32_t *roots, int32_t k,
+                  int32_t *depths_host, ent_batch_stats *stats);
+
+/* Allocates / frees device memory through the library (for callers without
+ * a device allocator of their own, e.g. ctypes). */
+int ent_device_alloc(ent_graph *g, int64_t bytes, void **out);
+int ent_device_free(ent_graph *g, void *p);
+
+/* 64-bit FNV-1a hash of each of `rows` rows of `cols` int32 values in device
+ * memory, computed on the device (for comparing large depth arrays without
+ * copying them). */
+int ent_device_row_hash(ent_graph *g, const int32_t *dev, int64_t rows,
+                        int64_t cols, uint64_t *hashes_host);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ENT_BATCH_H */

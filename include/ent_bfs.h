@@ -67,8 +67,11 @@ typedef struct ent_bfs_config {
   int32_t bu_chunk_words;  /* 64-vertex bitmap words per wavefront chunk in
                               the bottom-up sweep: 1..64 */
   int32_t graph_levels;    /* phases per captured graph / queued batch */
-  int32_t hub_cache;       /* LDS hub-set entries (0 = off) */
-  int32_t reserved[2];
+  int32_t hub_cache;       /* reserved (0) */
+  int32_t batch_alpha;     /* multi-source search: bottom-up when the joint
+                              frontier's degree sum * batch_alpha exceeds the
+                              degree sum of vertices still missing sources */
+  int32_t reserved;
 } ent_bfs_config;
 
 typedef struct ent_stats {
@@ -127,6 +130,11 @@ int ent_copy_to_host(ent_graph *g, const int32_t *src_dev, int32_t *dst_host,
 int ent_bfs_gpu_coalescing_mem(const int32_t *src_list, int32_t nsrc,
                                const int64_t *beg_pos, const int32_t *csr,
                                int64_t n, int64_t m, int32_t gpu_id);
+
+/* Runs a copy kernel that reads and writes `bytes` bytes (reps + 1 times) and
+ * reports the mean time per copy. Its hardware byte counters calibrate the
+ * HBM traffic figures of the search kernels. No reference counterpart. */
+int ent_calib_copy(int device, int64_t bytes, int32_t reps, double *ms_per_copy);
 
 /* Device self-test of the wavefront primitives (DPP inclusive scan, ballot
  * compaction, k-th set bit). Returns 0 when all lanes agree with the host. */

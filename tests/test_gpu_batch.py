@@ -1,0 +1,114 @@
+"""GPU parity of the multi-source BFS: every output row equals the
+single-source GPU result and the CPU oracle for its root."""
+import sys
+
+import numpy as np
+import pytest
+
+import graphs
+from enterprise_amd import DeviceGraph, HostGraph
+from enterprise_amd import _lib as L
+from enterprise_amd.bfs import row_hash_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def rmat16():
+    g = HostGraph.rmat(16, 16, seed=16)
+    dg = DeviceGraph(g)
+    yield g, dg
+    dg.free()
+
+
+def _check_batch(oracle, g, dg, roots, tag, oracle_rows=4):
+    _say(f"[gpu] batch {tag} k={len(roots)}")
+    out, st = dg.bfs_batch(roots)
+    assert out.shape == (len(roots), g.n)
+    for i, r in enumerate(roots):
+        want, _ = dg.bfs(int(r))
+        if not np.array_equal(out[i], want):
+            bad = np.flatnonzero(out[i] != want)
+            raise AssertionError(f"{tag}: source {i} root {r}: {bad.size} mismatches, first v={bad[0]} "
+                                 f"batch={out[i][bad[0]]} single={want[bad[0]]}")
+    for i in range(min(oracle_rows, len(roots))):
+        assert np.array_equal(out[i], oracle.bfs(g.beg_pos, g.adj, int(roots[i])))
+    deg = g.degrees()
+    reached = out >= 0
+    assert st["reached_vertices"] == int(reached.sum())
+    assert st["reached_edges"] == int((reached * deg[None, :]).sum())
+    assert st["levels"] == int(out.max()) + 1
+    return st
+
+
+@pytest.mark.parametrize("k", [1, 7, 64, 65, 130, 512])
+def test_batch_sizes(oracle, rmat16, k):
+    g, dg = rmat16
+    roots = g.pick_roots(k, 100 + k) if k <= 64 else np.r_[g.pick_roots(64, 5),
+                                                          g.pick_roots(k - 64, 6)]
+    _check_batch(oracle, g, dg, roots, f"rmat16/k{k}")
+
+
+def test_batch_uneven_and_repeated_roots(oracle, rmat16):
+    g, dg = rmat16
+    roots = g.pick_roots_mixed(64, 32, 1004)
+    roots = np.r_[roots, roots[:5]]           # repeats are allowed
+    _check_batch(oracle, g, dg, roots, "rmat16/mixed+repeats")
+
+
+@pytest.mark.parametrize("direction", ["auto", "topdown", "bottomup", "reference"])
+def test_batch_directions(oracle, rmat16, direction):
+    g, dg = rmat16
+    base = dg.config
+    d = {"auto": L.DIR_AUTO, "topdown": L.DIR_TOPDOWN, "bottomup": L.DIR_BOTTOMUP,
+         "reference": L.DIR_REFERENCE}[direction]
+    try:
+        dg.configure(direction=d, t_wave=4, t_block=24, t_grid=96, piece=64)
+        _check_batch(oracle, g, dg, g.pick_roots(96, 9), f"rmat16/{direction}/tiny-classes")
+    finally:
+        dg.set_config(base)
+
+
+@pytest.mark.parametrize("name", ["path300", "star5000", "grid37x23", "disjoint", "hypercube10"])
+def test_batch_closed_forms(oracle, name):
+    g, depth_fn = graphs.CLOSED_FORMS[name]()
+    roots = np.unique(np.r_[0, g.n // 2, g.n - 1, np.arange(0, g.n, max(1, g.n // 40))])[:64]
+    with DeviceGraph(g) as dg:
+        out, _ = dg.bfs_batch(roots)
+        for i, r in enumerate(roots):
+            assert np.array_equal(out[i], np.asarray(depth_fn(int(r)), dtype=np.int32)), (name, r)
+
+
+def test_batch_level_capacity_error(rmat16):
+    g, dg = rmat16
+    path, _ = graphs.path(300)
+    with DeviceGraph(path) as dp:
+        dp.batch_reserve(4, max_levels=10)
+        with pytest.raises(L.EntError) as e:
+            dp.bfs_batch([0, 1, 2, 3])
+        assert e.value.code == L.ENT_ENOMEM
+        dp.batch_reserve(4, max_levels=64)
+        with pytest.raises(L.EntError):
+            dp.bfs_batch([0, 299])          # needs 300 levels
+        dp.batch_reserve(4, max_levels=250)
+        out, _ = dp.bfs_batch([100, 150])      # 200 levels
+        assert out[0][299] == 199 and out[1][0] == 150
+
+
+def test_batch_device_hash(rmat16):
+    g, dg = rmat16
+    roots = g.pick_roots(32, 55)
+    ptr = dg.device_alloc(32 * g.n * 4)
+    try:
+        dg.bfs_batch_device(roots, ptr)
+        h = dg.row_hash(ptr, 32, g.n)
+        for i, r in enumerate(roots[:8]):
+            d, _ = dg.bfs(int(r))
+            assert h[i] == row_hash_host(d)
+    finally:
+        dg.device_free(ptr)
