@@ -105,3 +105,73 @@ def batch(args):
 
 
 MODES["batch"] = batch
+
+
+def grid(args):
+    """Config 3: 4096 x 4096 4-neighbour grid, 2% of edges deleted (seed
+    4096), largest component kept, 16 seeded roots. Compares the launch forms
+    on a search of ~8,000 levels: per-phase launches queued on a stream,
+    per-phase launches replayed from a captured hipGraph, one persistent
+    cooperative launch with grid barriers, and the host-synchronised loop of
+    the reference. Two roots are checked against the CPU oracle, which is
+    also the CPU baseline."""
+    import os
+    import sys
+
+    import numpy as np
+
+    from . import _lib as L
+    from .bfs import DeviceGraph
+    from .graph import HostGraph
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    side = int(getattr(args, "grid_side", 4096) or 4096)
+    t0 = time.perf_counter()
+    g = HostGraph.grid(side, side, delete_ppm=20000, seed=4096, keep_lcc=True)
+    roots = g.pick_roots(16, 1003)
+    log(f"[grid] {side}x{side}: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f}s)")
+    dg = DeviceGraph(g)
+    forms = {"stream": (L.LAUNCH_STREAM, 16), "graph": (L.LAUNCH_GRAPH, 16),
+             "persistent": (L.LAUNCH_PERSISTENT, 4), "levels": (L.LAUNCH_LEVELS, 2)}
+    res = {}
+    for name, (launch, nroots) in forms.items():
+        dg.configure(launch=launch, graph_levels=96)
+        dg.bfs_device(int(roots[0]))   # warm-up (graph capture, launch sizing)
+        times, levels, phases = [], [], []
+        for r in roots[:nroots]:
+            t1 = time.perf_counter()
+            st = dg.bfs_device(int(r))
+            times.append(time.perf_counter() - t1)
+            levels.append(st["levels"])
+            phases.append(st["phases"])
+        ms = float(np.mean(times)) * 1e3
+        res[name] = {"roots": nroots, "ms_per_bfs": round(ms, 3),
+                     "us_per_level": round(ms * 1e3 / float(np.mean(levels)), 3),
+                     "phases_per_level": round(float(np.mean(phases)) / float(np.mean(levels)), 3),
+                     "levels_mean": round(float(np.mean(levels)), 1)}
+        log(f"[grid] {name}: {res[name]}")
+    dg.configure(launch=L.LAUNCH_STREAM)
+    # CPU baseline and parity on two roots.
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import oracle_ct
+    oracle = oracle_ct.Oracle()
+    cpu_ms, mism = [], 0
+    for r in roots[:2]:
+        t1 = time.perf_counter()
+        want = oracle.bfs(g.beg_pos, g.adj, int(r))
+        cpu_ms.append((time.perf_counter() - t1) * 1e3)
+        got, _ = dg.bfs(int(r))
+        mism += int(not np.array_equal(got, want))
+    dg.free()
+    best = min(res, key=lambda k: res[k]["ms_per_bfs"])
+    return {"mode": "grid", "metric": "us per BFS level, high-diameter grid", "unit": "us/level",
+            "value": res[best]["us_per_level"], "best_form": best, "higher_is_better": False,
+            "config": {"workload": f"{side}x{side} 4-neighbour grid, 2% edges deleted (seed 4096), "
+                                   "largest component, 16 roots (seed 1003)", "n": g.n, "m": g.m},
+            "forms": res, "cpu_serial_ms_per_bfs": round(float(np.mean(cpu_ms)), 1),
+            "parity_roots": 2, "parity_mismatches": mism}
+
+
+MODES["grid"] = grid

@@ -771,7 +771,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
     *fin = b->h_ctl->st[(i - 1) & 1];
     if (fin->done) break;
     if (i > max_launches) return set_error(ENT_EINTERNAL, "batch did not finish in %lld launches", i);
-    kk = 2;
+    kk = std::min(2 * kk, 512);
   }
   b->expect_launches = std::min(fin->phases + 1, 512);
   g->last_grid = lgrid;
@@ -894,7 +894,7 @@ int ent_device_alloc(ent_graph *g, int64_t bytes, void **out) {
 }
 
 int ent_device_free(ent_graph *g, void *p) {
-  clear_error();
+  // Keeps the previous error text: callers free buffers on their error paths.
   if (!g) return set_error(ENT_EINVAL, "NULL graph");
   if (!p) return ENT_OK;
   HIP_TRY(hipSetDevice(g->device));

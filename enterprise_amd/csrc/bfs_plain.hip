@@ -53,14 +53,20 @@
 
 namespace ent {
 
-enum Phase : int { PH_TD = 0, PH_BU = 1, PH_INSPECT = 2 };
+enum Phase : int { PH_TD = 0, PH_BU = 1, PH_INSPECT = 2, PH_TDQ = 3, PH_REBUILD = 4 };
 enum : int { ERR_NONE = 0, ERR_LEVELS = 2, ERR_TIMEOUT = 4 };
 
 constexpr int kStripes = 64;     // counter / arrival stripes
 constexpr int kStripeU64 = 16;   // 128 B between stripes
 
+// Per-stripe counter words.
+enum : int { K_NF = 0, K_MF = 1, K_QN = 2, K_MAXDEG = 3, K_QOVF = 4 };
+constexpr int kQStage = 512;  // queue entries one block stages in LDS per phase
+
 struct Counters {
-  // stripe s: nf at [s * kStripeU64], mf at [s * kStripeU64 + 1]
+  // stripe s holds words [s * kStripeU64, s * kStripeU64 + 5): frontier
+  // vertices, degree sum, queue entries pushed to stripe s, max degree,
+  // queue overflow flag.
   unsigned long long v[kStripes * kStripeU64];
 };
 
@@ -79,6 +85,11 @@ struct State {
   unsigned long long total_mf;
   unsigned long long prev_nf;
   unsigned long long bu_mask;
+  int bm_valid;  // frontier of `level` is in front[level & 1]
+  int q_valid;   // frontier of `level` is in the striped queue qbuf[level & 1]
+  int pending;   // phase to run after PH_REBUILD
+  int pad2;
+  unsigned int qn[kStripes];  // entries per queue stripe
 };
 
 struct Control {
@@ -101,6 +112,7 @@ struct Args {
   unsigned long long *visited;
   unsigned long long *front[2];
   unsigned char *mark;
+  int *qbuf[2];  // striped frontier queues: stripe s at qbuf[p] + s * qcap_s
   Control *ctl;
   long long m;
   long long n_active;
@@ -112,6 +124,8 @@ struct Args {
   int mode, alpha, beta;
   int bu_probe;
   int chunk_words;
+  int qcap_s;   // entries per queue stripe
+  int q_small;  // largest frontier expanded from the queue
 };
 
 template <int BS>
@@ -121,6 +135,12 @@ struct Smem {
   unsigned long long found[NW][kWave];
   int excl[NW][kWave];
   unsigned long long nf, mf;
+  int qstage[kQStage];
+  int qcount;
+  int qovf;
+  int maxdeg;
+  int qbase;
+  int qpre[kStripes + 1];
   State S;
 };
 
@@ -128,15 +148,39 @@ __device__ __forceinline__ bool bit_of(const unsigned long long *bm, int v) {
   return ((bm[v >> 6] >> (v & 63)) & 1ull) != 0;
 }
 
-// Adds a block's frontier counts to its stripe.
+// Stages a newly found frontier vertex for the block's queue flush.
 template <int BS>
-__device__ __forceinline__ void add_counts(const Args &a, Smem<BS> &sm, int slot) {
+__device__ __forceinline__ void stage_push(Smem<BS> &sm, int v, long long deg) {
+  const int idx = atomicAdd(&sm.qcount, 1);
+  if (idx < kQStage) sm.qstage[idx] = v;
+  else sm.qovf = 1;
+  atomicMax(&sm.maxdeg, int(min(deg, (long long)INT32_MAX)));
+}
+
+// Adds a block's frontier counts to its stripe and appends its staged queue
+// entries to queue stripe (block % 64) of parity `par`.
+template <int BS>
+__device__ __forceinline__ void add_counts(const Args &a, Smem<BS> &sm, int slot, int par) {
   __syncthreads();
-  if (threadIdx.x == 0 && sm.nf) {
-    unsigned long long *c = a.ctl->cnt[slot].v + (blockIdx.x % kStripes) * kStripeU64;
-    atomicAdd(c, sm.nf);
-    atomicAdd(c + 1, sm.mf);
+  unsigned long long *c = a.ctl->cnt[slot].v + (blockIdx.x % kStripes) * kStripeU64;
+  if (threadIdx.x == 0) {
+    if (sm.nf) {
+      atomicAdd(c + K_NF, sm.nf);
+      atomicAdd(c + K_MF, sm.mf);
+      atomicMax(c + K_MAXDEG, (unsigned long long)sm.maxdeg);
+    }
+    const int k = min(sm.qcount, kQStage);
+    int base = 0;
+    if (k) base = int(atomicAdd(c + K_QN, (unsigned long long)k));
+    if (sm.qovf || base + k > a.qcap_s) atomicOr(c + K_QOVF, 1ull);
+    sm.qbase = base;
   }
+  __syncthreads();
+  const int k = min(sm.qcount, kQStage);
+  const int base = sm.qbase;
+  int *dst = a.qbuf[par] + (long long)(blockIdx.x % kStripes) * a.qcap_s;
+  for (int j = threadIdx.x; j < k; j += BS)
+    if (base + j < a.qcap_s) dst[base + j] = sm.qstage[j];
 }
 
 // Sums a per-lane count over the wavefront and adds it to the block total.
@@ -283,15 +327,17 @@ __device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
       if (mk) {
         a.mark[v] = 0;
         a.depth[v] = L + 1;
+        const long long d = a.off[v + 1] - a.off[v];
         nf += 1;
-        mf += (unsigned long long)(a.off[v + 1] - a.off[v]);
+        mf += (unsigned long long)d;
+        stage_push(sm, int(v), d);
       }
       if (lane == 0 && nw) a.visited[w] = vis | nw;
     }
     if (lane == 0) fn[w] = nw;
   }
   wave_add_counts(sm, nf, mf);
-  add_counts(a, sm, slot);
+  add_counts(a, sm, slot, (L + 1) & 1);
 }
 
 // Bottom-up level L: every unvisited vertex with neighbours looks for a
@@ -358,6 +404,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         a.depth[v] = L + 1;
         nf += 1;
         mf += (unsigned long long)(e - s);
+        stage_push(sm, v, e - s);
       }
       unsigned long long lmask = __ballot(k < total && !found && p < e);
       while (lmask) {
@@ -383,6 +430,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
           a.depth[vv] = L + 1;
           nf += 1;
           mf += (unsigned long long)(ee - ss);
+          stage_push(sm, vv, ee - ss);
         }
       }
     }
@@ -395,23 +443,107 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
     wave_sync();
   }
   wave_add_counts(sm, nf, mf);
-  add_counts(a, sm, slot);
+  add_counts(a, sm, slot, (L + 1) & 1);
+}
+
+// Top-down level L from the striped queue (small frontiers without
+// block-class vertices): one launch per level. A lane (or, for degree >=
+// t_wave, the whole wavefront) expands a frontier vertex; each unvisited
+// neighbour is claimed with an atomic OR on its visited word, and the
+// claiming lane writes its depth and stages it for the next queue.
+template <int BS>
+__device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
+  const int lane = lane_id();
+  if (threadIdx.x < kWave) {
+    const int c = int(sm.S.qn[lane] < unsigned(a.qcap_s) ? sm.S.qn[lane] : unsigned(a.qcap_s));
+    const int incl = wave_incl_scan(c);
+    sm.qpre[lane + 1] = incl;
+    if (lane == 0) sm.qpre[0] = 0;
+  }
+  __syncthreads();
+  const int total = sm.qpre[kStripes];
+  const int *qin = a.qbuf[L & 1];
+  unsigned long long nf = 0, mf = 0;
+  auto visit = [&](int u) {
+    const unsigned long long bit = 1ull << (u & 63);
+    unsigned long long *w = a.visited + (u >> 6);
+    if (*w & bit) return;
+    const unsigned long long old = atomicOr(w, bit);
+    if (old & bit) return;
+    a.depth[u] = L + 1;
+    const long long d = a.off[u + 1] - a.off[u];
+    nf += 1;
+    mf += (unsigned long long)d;
+    stage_push(sm, u, d);
+  };
+  const long long gt = (long long)blockIdx.x * BS + threadIdx.x;
+  const long long nt = (long long)gridDim.x * BS;
+  for (long long base = gt - lane; base < total; base += nt) {
+    const long long k = base + lane;
+    long long s = 0, e = 0;
+    if (k < total) {
+      int lo = 0, hi = kStripes - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sm.qpre[mid] <= k) lo = mid;
+        else hi = mid - 1;
+      }
+      const int v = qin[(long long)lo * a.qcap_s + (k - sm.qpre[lo])];
+      s = a.off[v];
+      e = a.off[v + 1];
+      if (e - s < a.t_wave)
+        for (long long i = s; i < e; ++i) visit(a.adj[i]);
+    }
+    unsigned long long wm = __ballot(k < total && e - s >= a.t_wave);
+    while (wm) {
+      const int src = __ffsll((long long)wm) - 1;
+      wm &= wm - 1;
+      const long long ss = __shfl(s, src), ee = __shfl(e, src);
+      for (long long i = ss + lane; i < ee; i += kWave) visit(a.adj[i]);
+    }
+  }
+  wave_add_counts(sm, nf, mf);
+  add_counts(a, sm, slot, (L + 1) & 1);
+}
+
+// Rebuilds front[L & 1] from the depth array (depth == L), one wavefront per
+// bitmap word, when a queue-driven level is followed by a bitmap-driven one.
+template <int BS>
+__device__ void rebuild_phase(const Args &a, Smem<BS> &sm, int L) {
+  unsigned long long *fr = a.front[L & 1];
+  const int lane = lane_id();
+  const long long nwv = (long long)gridDim.x * (BS / kWave);
+  for (long long w = xcd_wave_id(); w < a.nwords; w += nwv) {
+    const long long v = w * 64 + lane;
+    const unsigned long long word = __ballot(v < a.n && a.depth[v] == L);
+    if (lane == 0) fr[w] = word;
+  }
 }
 
 // Next state from the state after launch i - 1 and the frontier counts that
 // launch produced (nf, mf: vertices and degree sum of frontier level + 1).
 // Every block computes it the same way; nothing is shared but the inputs.
-__device__ State next_state(const Args &a, const State &p, unsigned long long nf, unsigned long long mf) {
-  State s = p;
-  if (p.done) return s;
-  s.phases = p.phases + 1;
-  if (p.phase == PH_TD) {  // the inspection of the same level comes next
+// Updates s (the state after launch i - 1, in LDS) in place to the state of
+// launch i. tot holds the summed counts of launch i - 1; q_fit says whether
+// every queue stripe stayed within its capacity (s.qn is filled by the
+// caller).
+__device__ __forceinline__ void next_state(const Args &a, State &s, const unsigned long long *tot, bool q_fit) {
+  if (s.done) return;
+  s.phases += 1;
+  const int pphase = s.phase;
+  if (pphase == PH_TD) {  // the inspection of the same level comes next
     s.phase = PH_INSPECT;
-    return s;
+    return;
   }
-  const int L = p.level;  // -1 before the first level
+  if (pphase == PH_REBUILD) {
+    s.phase = s.pending;
+    s.bm_valid = 1;
+    return;
+  }
+  const unsigned long long nf = tot[K_NF], mf = tot[K_MF];
+  const int L = s.level;  // -1 before the first level
   if (L >= 0) {
-    if (p.phase == PH_BU) {
+    if (pphase == PH_BU) {
       s.bu_levels += 1;
       if (L < 64) s.bu_mask |= 1ull << L;
     } else {
@@ -423,14 +555,14 @@ __device__ State next_state(const Args &a, const State &p, unsigned long long nf
   if (nf == 0) {
     s.levels = L + 1;
     s.done = 1;
-    return s;
+    return;
   }
   if (L + 1 >= a.n) {
     s.error |= ERR_LEVELS;
     s.done = 1;
-    return s;
+    return;
   }
-  const bool cur_bu = p.phase == PH_BU;
+  const bool cur_bu = pphase == PH_BU;
   bool next_bu;
   switch (a.mode) {
     case ENT_DIR_TOPDOWN: next_bu = false; break;
@@ -441,13 +573,27 @@ __device__ State next_state(const Args &a, const State &p, unsigned long long nf
     default: {
       const unsigned long long mu = (unsigned long long)a.m - s.total_mf;
       if (!cur_bu) next_bu = mf * (unsigned long long)a.alpha > mu;
-      else next_bu = !(nf * (unsigned long long)a.beta < (unsigned long long)a.n_active && nf < p.prev_nf);
+      else next_bu = !(nf * (unsigned long long)a.beta < (unsigned long long)a.n_active && nf < s.prev_nf);
     }
   }
+  // Frontier L + 1 is a bitmap after an inspection or a bottom-up level, and
+  // a complete queue (from any producer) unless a block or stripe overflowed.
+  const bool bm = pphase == PH_INSPECT || pphase == PH_BU;
+  bool q = pphase == PH_INSPECT || pphase == PH_TDQ || pphase == PH_BU;
+  if (!q_fit || tot[K_QOVF]) q = false;
   s.prev_nf = nf;
   s.level = L + 1;
-  s.phase = next_bu ? PH_BU : PH_TD;
-  return s;
+  s.bm_valid = bm ? 1 : 0;
+  s.q_valid = q ? 1 : 0;
+  if (next_bu) {
+    s.phase = bm ? PH_BU : PH_REBUILD;
+    s.pending = PH_BU;
+  } else if (q && nf <= (unsigned long long)a.q_small && tot[K_MAXDEG] < (unsigned long long)a.t_block) {
+    s.phase = PH_TDQ;
+  } else {
+    s.phase = bm ? PH_TD : PH_REBUILD;
+    s.pending = PH_TD;
+  }
 }
 
 // Launch prologue: the first wavefront sums the stripes of the previous
@@ -459,57 +605,60 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
     Control *c = a.ctl;
     const int lane = threadIdx.x;
     const Counters &pc = c->cnt[(i + 2) % 3];
-    unsigned long long nf = coherent ? ld_agent(&pc.v[lane * kStripeU64]) : pc.v[lane * kStripeU64];
-    unsigned long long mf = coherent ? ld_agent(&pc.v[lane * kStripeU64 + 1]) : pc.v[lane * kStripeU64 + 1];
+    const unsigned long long *src = &pc.v[lane * kStripeU64];
+    unsigned long long nf = coherent ? ld_agent(src + K_NF) : src[K_NF];
+    unsigned long long mf = coherent ? ld_agent(src + K_MF) : src[K_MF];
+    const unsigned long long qn = coherent ? ld_agent(src + K_QN) : src[K_QN];
+    unsigned long long md = coherent ? ld_agent(src + K_MAXDEG) : src[K_MAXDEG];
+    unsigned long long ovf = coherent ? ld_agent(src + K_QOVF) : src[K_QOVF];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       nf += __shfl_xor(nf, o);
       mf += __shfl_xor(mf, o);
+      md = max(md, __shfl_xor(md, o));
+      ovf |= __shfl_xor(ovf, o);
     }
+    __shared__ unsigned long long tot[8];
     if (lane == 0) {
-      State prev;
-      if (coherent) {
-        // Word-wise agent-scope loads: the previous state was written by
-        // another block inside this launch.
-        const unsigned *src = reinterpret_cast<const unsigned *>(&c->st[(i + 1) & 1]);
-        unsigned *dst = reinterpret_cast<unsigned *>(&prev);
-        for (int w = 0; w < int(sizeof(State) / 4); ++w) dst[w] = ld_agent(src + w);
-      } else {
-        prev = c->st[(i + 1) & 1];
-      }
-      const State S = next_state(a, prev, nf, mf);
-      sm.S = S;
+      tot[K_NF] = nf;
+      tot[K_MF] = mf;
+      tot[K_MAXDEG] = md;
+      tot[K_QOVF] = ovf;
+    }
+    const bool q_fit = __ballot(qn > (unsigned long long)a.qcap_s) == 0;
+    // The previous state (written by block 0 of launch i - 1, or by another
+    // block inside a persistent launch) is copied word by word into LDS.
+    {
+      const unsigned *s32 = reinterpret_cast<const unsigned *>(&c->st[(i + 1) & 1]);
+      unsigned *d32 = reinterpret_cast<unsigned *>(&sm.S);
+      for (int w = lane; w < int(sizeof(State) / 4); w += kWave) d32[w] = coherent ? ld_agent(s32 + w) : s32[w];
+    }
+    wave_sync();
+    // Queue stripe sizes of the frontier just produced; only meaningful when
+    // the next phase reads the queue (launches after a phase that writes it).
+    const int pphase = sm.S.phase;
+    if (!sm.S.done && (pphase == PH_INSPECT || pphase == PH_BU || pphase == PH_TDQ))
+      sm.S.qn[lane] = unsigned(min(qn, (unsigned long long)a.qcap_s));
+    wave_sync();
+    if (lane == 0) {
+      next_state(a, sm.S, tot, q_fit);
       sm.nf = sm.mf = 0;
-      if (blockIdx.x == 0) {
-        const unsigned *src = reinterpret_cast<const unsigned *>(&S);
-        unsigned *dst = reinterpret_cast<unsigned *>(&c->st[i & 1]);
-        for (int w = 0; w < int(sizeof(State) / 4); ++w) st_agent(dst + w, src[w]);
-      }
+      sm.qcount = 0;
+      sm.qovf = 0;
+      sm.maxdeg = 0;
+    }
+    wave_sync();
+    if (blockIdx.x == 0) {
+      const unsigned *s32 = reinterpret_cast<const unsigned *>(&sm.S);
+      unsigned *d32 = reinterpret_cast<unsigned *>(&c->st[i & 1]);
+      for (int w = lane; w < int(sizeof(State) / 4); w += kWave) st_agent(d32 + w, s32[w]);
     }
     if (blockIdx.x == 0) {
-      Counters &z = c->cnt[(i + 1) % 3];
-      st_agent(&z.v[lane * kStripeU64], 0ull);
-      st_agent(&z.v[lane * kStripeU64 + 1], 0ull);
+      unsigned long long *z = &c->cnt[(i + 1) % 3].v[lane * kStripeU64];
+      for (int w = 0; w < 5; ++w) st_agent(z + w, 0ull);
     }
   }
   __syncthreads();
-}
-
-// Two-level arrival: blocks count on their stripe, the last of each stripe on
-// arrive_all; returns true for the single last block. Thread 0 only.
-__device__ __forceinline__ bool arrive_last(Control *c) {
-  const unsigned G = gridDim.x;
-  const unsigned s = blockIdx.x % kStripes;
-  const unsigned quota = (G - s + kStripes - 1) / kStripes;
-  const unsigned nstripes = G < kStripes ? G : kStripes;
-  unsigned *sc = &c->arrive[s * 32];
-  const unsigned t = __hip_atomic_fetch_add(sc, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  if (t != quota - 1) return false;
-  st_agent(sc, 0u);
-  const unsigned t2 = __hip_atomic_fetch_add(&c->arrive_all, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-  if (t2 != nstripes - 1) return false;
-  st_agent(&c->arrive_all, 0u);
-  return true;
 }
 
 // depth = -1 (root 0), visited = isolated vertices, padding bits and the root,
@@ -531,8 +680,15 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu) {
   Control *c = a.ctl;
   constexpr long long kC = (long long)kStripes * kStripeU64;
   const unsigned long long root_deg = (unsigned long long)(a.off[root + 1] - a.off[root]);
-  for (long long i = gt; i < 3 * kC; i += nt)
-    c->cnt[i / kC].v[i % kC] = i == 2 * kC ? 1ull : (i == 2 * kC + 1 ? root_deg : 0ull);
+  for (long long i = gt; i < 3 * kC; i += nt) {
+    unsigned long long x = 0;
+    if (i >= 2 * kC && i < 2 * kC + 5) {
+      const int w = int(i - 2 * kC);
+      x = w == K_NF ? 1ull : w == K_MF ? root_deg : w == K_QN ? 1ull : w == K_MAXDEG ? root_deg : 0ull;
+    }
+    c->cnt[i / kC].v[i % kC] = x;
+  }
+  if (gt == 0) a.qbuf[0][0] = root;
   for (long long i = gt; i < kStripes * 32; i += nt) c->arrive[i] = 0;
   if (gt == 0) {
     State S{};
@@ -549,9 +705,13 @@ template <int BS>
 __device__ __forceinline__ void run_phase(const Args &a, Smem<BS> &sm, long long i) {
   const int L = sm.S.level;
   const int slot = int(i % 3);
-  if (sm.S.phase == PH_TD) td_phase<BS>(a, sm, L);
-  else if (sm.S.phase == PH_BU) bu_phase<BS>(a, sm, L, slot);
-  else inspect_phase<BS>(a, sm, L, slot);
+  switch (sm.S.phase) {
+    case PH_TD: td_phase<BS>(a, sm, L); break;
+    case PH_BU: bu_phase<BS>(a, sm, L, slot); break;
+    case PH_INSPECT: inspect_phase<BS>(a, sm, L, slot); break;
+    case PH_TDQ: tdq_phase<BS>(a, sm, L, slot); break;
+    default: rebuild_phase<BS>(a, sm, L); break;
+  }
 }
 
 // Launch i of a search: one phase. No end-of-launch synchronisation: the next
@@ -564,12 +724,34 @@ __global__ __launch_bounds__(BS) void k_level(Args a, long long i) {
   run_phase<BS>(a, sm, i);
 }
 
-// Grid barrier (two-level arrival, generation flag). Bounded: a wait longer
-// than ~4 s marks the search failed and releases the waiter.
+// Two-level arrival: blocks count on their stripe, the last of each stripe on
+// arrive_all; returns true for the single last block. Thread 0 only.
+__device__ __forceinline__ bool arrive_last(Control *c) {
+  const unsigned G = gridDim.x;
+  const unsigned s = blockIdx.x % kStripes;
+  const unsigned quota = (G - s + kStripes - 1) / kStripes;
+  const unsigned nstripes = G < kStripes ? G : kStripes;
+  unsigned *sc = &c->arrive[s * 32];
+  const unsigned t = __hip_atomic_fetch_add(sc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t != quota - 1) return false;
+  st_agent(sc, 0u);
+  const unsigned t2 = __hip_atomic_fetch_add(&c->arrive_all, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t2 != nstripes - 1) return false;
+  st_agent(&c->arrive_all, 0u);
+  return true;
+}
+
+// Grid barrier (two-level arrival, generation flag). Every wavefront first
+// waits for its own stores to complete (the block barrier alone does not), so
+// one agent-scope release per block (L2 write-back) publishes the whole
+// block's writes; after the generation flips, one agent-scope acquire per
+// block (L1/L2 invalidate) precedes the block's next reads. Bounded: a wait
+// longer than ~4 s marks the search failed and releases the waiter.
 __device__ __forceinline__ void grid_barrier(const Args &a) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are done
   __syncthreads();
   if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     Control *c = a.ctl;
     const unsigned g = ld_agent(&c->gen);
     if (arrive_last(c)) {
@@ -585,9 +767,9 @@ __device__ __forceinline__ void grid_barrier(const Args &a) {
         }
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // Whole search in one cooperative launch: launches 0, 1, 2, ... separated by
@@ -646,6 +828,8 @@ void release(ent_graph *g) {
   ignore(hipFree(g->front[0]));
   ignore(hipFree(g->front[1]));
   ignore(hipFree(g->mark));
+  ignore(hipFree(g->qbuf[0]));
+  ignore(hipFree(g->qbuf[1]));
   ignore(hipFree(g->ctl));
   if (g->h_ctl) ignore(hipHostFree(g->h_ctl));
   if (g->ev0) ignore(hipEventDestroy(g->ev0));
@@ -727,6 +911,10 @@ Args make_args(ent_graph *g, int *depth) {
   a.front[0] = g->front[0];
   a.front[1] = g->front[1];
   a.mark = g->mark;
+  a.qbuf[0] = g->qbuf[0];
+  a.qbuf[1] = g->qbuf[1];
+  a.qcap_s = g->qcap_s;
+  a.q_small = int(std::min<int64_t>(std::max<int64_t>(4096, g->n / 256), int64_t(kStripes) * g->qcap_s / 2));
   a.ctl = g->ctl;
   a.m = g->m;
   a.n_active = g->n_active;
@@ -863,9 +1051,10 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
         if (rc) return rc;
         if (fin->done) break;
         if (i > max_launches) return set_error(ENT_EINTERNAL, "search did not finish in %lld launches", i);
-        if (cfg.launch == ENT_LAUNCH_STREAM) k = std::max(2, cfg.graph_levels / 4);
+        // Follow-up batches double (deep searches need thousands of phases).
+        if (cfg.launch == ENT_LAUNCH_STREAM) k = std::min(2 * k, 1024);
       }
-      if (cfg.launch == ENT_LAUNCH_STREAM) g->expect_launches = int(std::min<int64_t>(fin->phases + 1, 256));
+      if (cfg.launch == ENT_LAUNCH_STREAM) g->expect_launches = int(std::min<int64_t>(fin->phases + 1, 1 << 16));
       return ENT_OK;
     }
     case ENT_LAUNCH_GRAPH: {
@@ -1008,6 +1197,9 @@ int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int6
   UP_ALLOC(g->front[0], sizeof(unsigned long long) * size_t(g->nwords));
   UP_ALLOC(g->front[1], sizeof(unsigned long long) * size_t(g->nwords));
   UP_ALLOC(g->mark, size_t(n));
+  g->qcap_s = 4096;
+  UP_ALLOC(g->qbuf[0], sizeof(int) * size_t(kStripes) * size_t(g->qcap_s));
+  UP_ALLOC(g->qbuf[1], sizeof(int) * size_t(kStripes) * size_t(g->qcap_s));
   UP_ALLOC(g->ctl, sizeof(Control));
   UP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_ctl), sizeof(Control), hipHostMallocDefault));
   UP_TRY(hipMemcpy(g->off, beg_pos, sizeof(long long) * size_t(n + 1), hipMemcpyHostToDevice));

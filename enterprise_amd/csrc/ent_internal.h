@@ -37,6 +37,8 @@ struct ent_graph {
   unsigned long long *visited = nullptr;
   unsigned long long *front[2] = {nullptr, nullptr};
   unsigned char *mark = nullptr;
+  int *qbuf[2] = {nullptr, nullptr};  // striped frontier queues (kQStripes x qcap_s)
+  int qcap_s = 0;
   ent::Control *ctl = nullptr;
   ent::Control *h_ctl = nullptr;
   ent_bfs_config cfg{};
