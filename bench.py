@@ -187,6 +187,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--check-roots", type=int, default=4)
+    ap.add_argument("--serve-scale", type=int, default=26)
+    ap.add_argument("--serve-budget-gb", type=int, default=160)
+    ap.add_argument("--rates", default="100,200,500,1000,2000,5000")
+    ap.add_argument("--duration", type=float, default=2.0)
+    ap.add_argument("--grid-side", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "

@@ -90,6 +90,7 @@ SIGNATURES = [
     ("ent_graph_upload", C.c_int, [i64p, i32p, C.c_int64, C.c_int64, C.c_int,
                                    C.POINTER(C.c_void_p)]),
     ("ent_graph_free", C.c_int, [C.c_void_p]),
+    ("ent_graph_share", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("ent_graph_info", C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p]),
     ("ent_set_config", C.c_int, [C.c_void_p, C.POINTER(BfsConfig)]),
     ("ent_get_config", C.c_int, [C.c_void_p, C.POINTER(BfsConfig)]),

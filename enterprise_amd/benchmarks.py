@@ -175,3 +175,86 @@ def grid(args):
 
 
 MODES["grid"] = grid
+
+
+def serve(args):
+    """Config 5: a continuous-batching server over two resident graphs, R-MAT
+    scale 26 (80% of queries) and scale 20, queries arriving as a seeded
+    Poisson stream at each rate, uniform non-isolated roots, up to 64 queries
+    per multi-source batch. Reports p50/p99 latency (submit to end of the
+    serving batch) and GTEPS per rate; sampled queries are checked against
+    the single-source path by device row checksums."""
+    import sys
+
+    import numpy as np
+
+    from .bfs import DeviceGraph
+    from .graph import HostGraph
+    from .serving import BFSServer, run_load
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    big_scale = int(getattr(args, "serve_scale", 26) or 26)
+    rates = [int(x) for x in str(getattr(args, "rates", "100,200,500,1000,2000,5000")).split(",")]
+    dur = float(getattr(args, "duration", 2.0) or 2.0)
+    t0 = time.perf_counter()
+    gb = HostGraph.rmat(big_scale, 16, seed=big_scale)
+    log(f"[serve] R-MAT {big_scale}: n={gb.n} m={gb.m} ({time.perf_counter() - t0:.1f}s)")
+    gs = HostGraph.rmat(20, 16, seed=20)
+    roots = {"large": gb.pick_roots(4096, 1005), "small": gs.pick_roots(4096, 1006)}
+    db, ds = DeviceGraph(gb), DeviceGraph(gs)
+    from . import autotune
+    tuned = autotune.load("rmat22")
+    if tuned:
+        for d in (db, ds):
+            d.configure(**{k: v for k, v in tuned.items() if k not in ("reserved", "launch")})
+    m_big = gb.m
+    del gb  # the host copy is no longer needed
+    srv = BFSServer(mem_budget_bytes=int(getattr(args, "serve_budget_gb", 160)) << 30, max_batch=64,
+                    max_levels=28, max_slots=4)
+    slots_big = srv.add_graph("large", db, share_budget=0.9)
+    slots_small = srv.add_graph("small", ds, share_budget=0.1)
+    log(f"[serve] slots: large {slots_big}, small {slots_small}")
+    srv.start()
+    run_load(srv, 200, 0.3, 1, roots, {"large": 0.8, "small": 0.2})   # warm-up
+    per_rate = []
+    try:
+        for rate in rates:
+            for lane in srv.lanes.values():
+                lane.stats.update({"batches": 0, "queries": 0, "reached_edges": 0, "busy_s": 0.0,
+                                   "batch_sizes": []})
+            t1 = time.perf_counter()
+            res = run_load(srv, rate, dur, 1005 + rate, roots, {"large": 0.8, "small": 0.2})
+            wall = time.perf_counter() - t1
+            lat = np.array([q.latency_ms for q in res])
+            edges = sum(l.stats["reached_edges"] for l in srv.lanes.values())
+            sizes = sum((l.stats["batch_sizes"] for l in srv.lanes.values()), [])
+            # Sampled check against the single-source path.
+            rng = np.random.default_rng(rate)
+            mism = 0
+            sample = rng.choice(len(res), size=min(8, len(res)), replace=False)
+            for i in sample:
+                q = res[int(i)]
+                d = db if q.graph == "large" else ds
+                d.bfs_device(q.root)
+                mism += int(int(d.row_hash(d.depth_device_ptr(), 1, d.n)[0]) != q.row_hash)
+            row = {"rate_qps": rate, "queries": len(res), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                   "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                   "throughput_qps": round(len(res) / wall, 1), "gteps": round(edges / wall / 1e9, 2),
+                   "mean_batch": round(float(np.mean(sizes)), 1) if sizes else 0,
+                   "checked": int(len(sample)), "mismatches": mism}
+            log(f"[serve] {row}")
+            per_rate.append(row)
+    finally:
+        srv.close()
+        db.free()
+        ds.free()
+    return {"mode": "serve", "metric": "p50/p99 query latency vs GTEPS", "unit": "ms",
+            "config": {"workload": f"serving R-MAT {big_scale} (80%) + R-MAT 20, Poisson arrivals, "
+                                   "64-source continuous batching", "m_large": int(m_big),
+                       "slots": {"large": slots_big, "small": slots_small}, "duration_s": dur},
+            "rates": per_rate}
+
+
+MODES["serve"] = serve

@@ -24,7 +24,14 @@ def default_config():
 class DeviceGraph:
     """A CSR graph resident in HBM plus its BFS workspace."""
 
-    def __init__(self, host_graph=None, beg_pos=None, adj=None, device=0):
+    def __init__(self, host_graph=None, beg_pos=None, adj=None, device=0, share=None):
+        if share is not None:
+            # Second handle on share's resident CSR, with its own workspace.
+            h = C.c_void_p()
+            check(lib().ent_graph_share(share.handle, C.byref(h)))
+            self._h = h
+            self.n, self.m, self.device = share.n, share.m, share.device
+            return
         if host_graph is not None:
             beg_pos, adj = host_graph.beg_pos, host_graph.adj
         b = np.ascontiguousarray(beg_pos, dtype=np.int64)

@@ -741,8 +741,8 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   std::sort(vs.begin(), vs.end());
   vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
   unsigned long long mf = 0, alg = 0;
-  for (int32_t v : vs) mf += (unsigned long long)g->h_deg[size_t(v)];
-  for (auto &p : vw) alg += 8ull + 12ull * (unsigned long long)g->h_deg[size_t(p.first)];
+  for (int32_t v : vs) mf += (unsigned long long)g->store->h_deg[size_t(v)];
+  for (auto &p : vw) alg += 8ull + 12ull * (unsigned long long)g->store->h_deg[size_t(p.first)];
   unsigned long long munf = (unsigned long long)g->m;  // every vertex still needs sources
   HIP_TRY(hipMemcpyAsync(b->roots, roots, sizeof(int32_t) * size_t(k), hipMemcpyHostToDevice, g->stream));
   const int grid = g->cus * 8;

@@ -817,9 +817,7 @@ void release(ent_graph *g) {
   batch_release(g);
   if (g->gexec) ignore(hipGraphExecDestroy(g->gexec));
   if (g->graph) ignore(hipGraphDestroy(g->graph));
-  ignore(hipFree(g->off));
-  ignore(hipFree(g->adj));
-  ignore(hipFree(g->nzmask));
+  g->store.reset();
   ignore(hipFree(g->hub_block));
   ignore(hipFree(g->piece_v));
   ignore(hipFree(g->piece_k));
@@ -845,7 +843,7 @@ int build_lists(ent_graph *g) {
   if (g->lists_t_block == c.t_block && g->lists_t_grid == c.t_grid && g->lists_piece == c.piece) return ENT_OK;
   std::vector<int32_t> hubs, pv, pk;
   for (int64_t v = 0; v < g->n; ++v) {
-    const int32_t d = g->h_deg[size_t(v)];
+    const int32_t d = g->store->h_deg[size_t(v)];
     if (d >= c.t_grid) {
       const int32_t np = int32_t((int64_t(d) + c.piece - 1) / c.piece);
       for (int32_t k = 0; k < np; ++k) {
@@ -1095,9 +1093,63 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
   return ENT_OK;
 }
 
+// Per-handle search workspace: stream, events, depth, bitmaps, marks,
+// queues and control block.
+int alloc_workspace(ent_graph *g) {
+  int rc;
+  const int64_t n = g->n;
+  HIP_TRY(hipSetDevice(g->device));
+  HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&g->ev0));
+  HIP_TRY(hipEventCreate(&g->ev1));
+  HIP_TRY(hipDeviceGetAttribute(&g->cus, hipDeviceAttributeMultiprocessorCount, g->device));
+  g->qcap_s = 4096;
+  if ((rc = dev_alloc(g, reinterpret_cast<void **>(&g->depth), sizeof(int) * size_t(n))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&g->visited), sizeof(unsigned long long) * size_t(g->nwords))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&g->front[0]), sizeof(unsigned long long) * size_t(g->nwords))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&g->front[1]), sizeof(unsigned long long) * size_t(g->nwords))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&g->mark), size_t(n))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&g->qbuf[0]), sizeof(int) * size_t(kStripes) * size_t(g->qcap_s))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&g->qbuf[1]), sizeof(int) * size_t(kStripes) * size_t(g->qcap_s))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&g->ctl), sizeof(Control))))
+    return rc;
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_ctl), sizeof(Control), hipHostMallocDefault));
+  HIP_TRY(hipMemset(g->ctl, 0, sizeof(Control)));
+  HIP_TRY(hipDeviceSynchronize());
+  return build_lists(g);
+}
+
 }  // namespace
 
 extern "C" {
+
+int ent_graph_share(ent_graph *src, ent_graph **out) {
+  clear_error();
+  if (!src || !out) return set_error(ENT_EINVAL, "NULL argument");
+  *out = nullptr;
+  ent_graph *g = new (std::nothrow) ent_graph();
+  if (!g) return set_error(ENT_ENOMEM, "out of host memory");
+  {
+    std::lock_guard<std::mutex> lk(src->mu);
+    g->device = src->device;
+    g->n = src->n;
+    g->m = src->m;
+    g->n_active = src->n_active;
+    g->nwords = src->nwords;
+    g->store = src->store;
+    g->off = src->off;
+    g->adj = src->adj;
+    g->nzmask = src->nzmask;
+    g->cfg = src->cfg;
+  }
+  const int rc = alloc_workspace(g);
+  if (rc) {
+    release(g);
+    return rc;
+  }
+  *out = g;
+  return ENT_OK;
+}
 
 void ent_bfs_default_config(ent_bfs_config *c) {
   if (!c) return;
@@ -1173,47 +1225,37 @@ int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int6
     }                                                                  \
   } while (0)
   UP_TRY(hipSetDevice(device));
-  UP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
-  UP_TRY(hipEventCreate(&g->ev0));
-  UP_TRY(hipEventCreate(&g->ev1));
-  UP_TRY(hipDeviceGetAttribute(&g->cus, hipDeviceAttributeMultiprocessorCount, device));
   std::vector<unsigned long long> nz(size_t(g->nwords), 0ull);
-  g->h_deg.resize(size_t(n));
+  g->store = std::make_shared<GraphStore>();
+  g->store->device = device;
+  std::vector<int32_t> &hdeg = g->store->h_deg;
+  hdeg.resize(size_t(n));
   int64_t active = 0;
   for (int64_t v = 0; v < n; ++v) {
     const int64_t d = beg_pos[v + 1] - beg_pos[v];
-    g->h_deg[size_t(v)] = int32_t(d);
+    hdeg[size_t(v)] = int32_t(d);
     if (d > 0) {
       nz[size_t(v >> 6)] |= 1ull << (v & 63);
       ++active;
     }
   }
   g->n_active = active;
-  UP_ALLOC(g->off, sizeof(long long) * size_t(n + 1));
-  UP_ALLOC(g->adj, sizeof(int) * size_t(m));
-  UP_ALLOC(g->nzmask, sizeof(unsigned long long) * size_t(g->nwords));
-  UP_ALLOC(g->depth, sizeof(int) * size_t(n));
-  UP_ALLOC(g->visited, sizeof(unsigned long long) * size_t(g->nwords));
-  UP_ALLOC(g->front[0], sizeof(unsigned long long) * size_t(g->nwords));
-  UP_ALLOC(g->front[1], sizeof(unsigned long long) * size_t(g->nwords));
-  UP_ALLOC(g->mark, size_t(n));
-  g->qcap_s = 4096;
-  UP_ALLOC(g->qbuf[0], sizeof(int) * size_t(kStripes) * size_t(g->qcap_s));
-  UP_ALLOC(g->qbuf[1], sizeof(int) * size_t(kStripes) * size_t(g->qcap_s));
-  UP_ALLOC(g->ctl, sizeof(Control));
-  UP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_ctl), sizeof(Control), hipHostMallocDefault));
+  UP_ALLOC(g->store->off, sizeof(long long) * size_t(n + 1));
+  UP_ALLOC(g->store->adj, sizeof(int) * size_t(m));
+  UP_ALLOC(g->store->nzmask, sizeof(unsigned long long) * size_t(g->nwords));
+  g->off = g->store->off;
+  g->adj = g->store->adj;
+  g->nzmask = g->store->nzmask;
   UP_TRY(hipMemcpy(g->off, beg_pos, sizeof(long long) * size_t(n + 1), hipMemcpyHostToDevice));
   if (m) UP_TRY(hipMemcpy(g->adj, adj, sizeof(int) * size_t(m), hipMemcpyHostToDevice));
   UP_TRY(hipMemcpy(g->nzmask, nz.data(), sizeof(unsigned long long) * size_t(g->nwords), hipMemcpyHostToDevice));
-  UP_TRY(hipMemset(g->ctl, 0, sizeof(Control)));
-  UP_TRY(hipDeviceSynchronize());
-#undef UP_TRY
-#undef UP_ALLOC
-  rc = build_lists(g);
+  rc = alloc_workspace(g);
   if (rc) {
     release(g);
     return rc;
   }
+#undef UP_TRY
+#undef UP_ALLOC
   *out = g;
   return ENT_OK;
 }

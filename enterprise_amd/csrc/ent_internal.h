@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -18,21 +19,38 @@ struct Control;
 struct BatchWorkspace;
 template <class T>
 inline void ignore(T) {}
+
+// Resident CSR shared by every handle made with ent_graph_share; freed with
+// the last handle.
+struct GraphStore {
+  int device = 0;
+  std::shared_ptr<ent::GraphStore> store;  // resident CSR (shared)
+  long long *off = nullptr;                 // = store->off
+  int *adj = nullptr;                       // = store->adj
+  unsigned long long *nzmask = nullptr;     // = store->nzmask
+  std::vector<int32_t> h_deg;  // host copy of degrees (for the static lists)
+  ~GraphStore() {
+    if (hipSetDevice(device) != hipSuccess) return;
+    ignore(hipFree(off));
+    ignore(hipFree(adj));
+    ignore(hipFree(nzmask));
+  }
+};
 }  // namespace ent
 
 struct ent_graph {
   int device = 0;
   int64_t n = 0, m = 0, n_active = 0, nwords = 0;
   hipStream_t stream = nullptr;
-  long long *off = nullptr;
-  int *adj = nullptr;
-  unsigned long long *nzmask = nullptr;
+  std::shared_ptr<ent::GraphStore> store;  // resident CSR (shared)
+  long long *off = nullptr;                 // = store->off
+  int *adj = nullptr;                       // = store->adj
+  unsigned long long *nzmask = nullptr;     // = store->nzmask
   int *hub_block = nullptr;
   int *piece_v = nullptr;
   int *piece_k = nullptr;
   int n_hub_block = 0, n_pieces = 0;
   int lists_t_block = -1, lists_t_grid = -1, lists_piece = -1;
-  std::vector<int32_t> h_deg;  // host copy of degrees (for the static lists)
   int *depth = nullptr;
   unsigned long long *visited = nullptr;
   unsigned long long *front[2] = {nullptr, nullptr};

@@ -100,6 +100,13 @@ void ent_bfs_default_config(ent_bfs_config *cfg);
 int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n,
                      int64_t m, int device, ent_graph **out);
 int ent_graph_free(ent_graph *g);
+
+/* A second handle on the same resident graph: it shares src's device CSR
+ * (no copy; the CSR is freed with the last handle) and has its own stream
+ * and search workspace, so searches on the two handles can run concurrently
+ * on the device. No reference counterpart (the reference holds one graph and
+ * one search per process, comm.h:137-158). */
+int ent_graph_share(ent_graph *src, ent_graph **out);
 int ent_graph_info(const ent_graph *g, int64_t *n, int64_t *m,
                    int64_t *n_active, int64_t *device_bytes);
 

