@@ -81,7 +81,7 @@ def load_tuned(dg, name):
     if os.path.exists(path):
         with open(path) as f:
             cfg = json.load(f)
-        dg.configure(**{k: v for k, v in cfg.get("config", cfg).items() if k not in ("reserved", "launch")})
+        dg.configure(**{k: v for k, v in cfg.get("config", cfg).items() if k not in ("reserved", "launch", "hub_cache")})
         return path
     return None
 
@@ -98,6 +98,8 @@ def bench_plain(args):
     roots = g.pick_roots(64, 1002)
     dg = DeviceGraph(g, device=0)
     tuned = load_tuned(dg, f"rmat{scale}")
+    if args.set:
+        dg.configure(**{kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.set})
     if args.launch is not None:
         dg.configure(launch={"persistent": L.LAUNCH_PERSISTENT, "levels": L.LAUNCH_LEVELS,
                              "graph": L.LAUNCH_GRAPH, "stream": L.LAUNCH_STREAM}[args.launch])
@@ -192,6 +194,8 @@ def main():
     ap.add_argument("--rates", default="100,200,500,1000,2000,5000")
     ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--grid-side", type=int, default=4096)
+    ap.add_argument("--set", action="append", default=[],
+                    help="config override key=value (ent_bfs_config field), repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "

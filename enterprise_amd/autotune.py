@@ -36,6 +36,7 @@ SPACE = {
     "bu_chunk_words": [2, 4, 8, 16, 32],
     "alpha": [4, 8, 15, 30, 60],
     "beta": [6, 12, 18, 36, 72],
+    "queue_small": [-1, 1024, 4096, 16384, 65536],
 }
 
 

@@ -57,7 +57,7 @@ def batch(args):
     from . import autotune
     tuned = autotune.load(f"rmat{scale}") or autotune.load("rmat22")
     if tuned:
-        dg.configure(**{kk: v for kk, v in tuned.items() if kk not in ("reserved", "launch")})
+        dg.configure(**{kk: v for kk, v in tuned.items() if kk not in ("reserved", "launch", "hub_cache")})
     # Plain path on the same roots (host window per root, as bench.py plain).
     plain_rates, plain_edges, plain_time, plain_hash = [], 0, 0.0, []
     for r in roots[:8]:
@@ -208,7 +208,7 @@ def serve(args):
     tuned = autotune.load("rmat22")
     if tuned:
         for d in (db, ds):
-            d.configure(**{k: v for k, v in tuned.items() if k not in ("reserved", "launch")})
+            d.configure(**{k: v for k, v in tuned.items() if k not in ("reserved", "launch", "hub_cache")})
     m_big = gb.m
     del gb  # the host copy is no longer needed
     srv = BFSServer(mem_budget_bytes=int(getattr(args, "serve_budget_gb", 160)) << 30, max_batch=64,

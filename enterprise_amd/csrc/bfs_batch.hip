@@ -394,6 +394,7 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
 template <int BS, int W>
 __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const unsigned long long *fs = level_ptr(a, L);
+  const unsigned long long *jf = a.jf[L & 1];  // joint frontier of level L
   unsigned long long *nx = level_ptr(a, L + 1);
   unsigned long long *jn = a.jf[(L + 1) & 1];
   const int lane = lane_id();
@@ -425,7 +426,9 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     if (need) {
       const long long pe = min(e, s + a.probe);
       for (p = s; p < pe; ++p) {
-        const Mask<W> f = load_mask<W>(fs, a.adj[p]);
+        const int u = a.adj[p];
+        if (!bitv(jf, u)) continue;  // no source has u in its level-L frontier
+        const Mask<W> f = load_mask<W>(fs, u);
         bool all = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -456,7 +459,10 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
         Mask<W> f;
 #pragma unroll
         for (int w = 0; w < W; ++w) f.w[w] = 0;
-        if (q < ee) f = load_mask<W>(fs, a.adj[q]);
+        if (q < ee) {
+          const int u = a.adj[q];
+          if (bitv(jf, u)) f = load_mask<W>(fs, u);
+        }
         bool all = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -484,17 +490,19 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
 
 // Output: per 64-vertex segment, replay the level masks of each 64-source
 // group into an LDS table of levels (u8, 0xFF = unreached) and store the 64
-// rows of the group with 256-byte coalesced writes.
+// rows of the group with 256-byte coalesced writes. The wavefronts of a block
+// share one segment (wavefront w takes groups w, w + 4, ...), so the 64-byte
+// mask line of a vertex is fetched once and read by every group while hot.
 template <int BS, int W>
 __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
+  constexpr int NWV = BS / kWave;
   const int wid = threadIdx.x / kWave, lane = lane_id();
   unsigned char *d8 = sm.d8[wid];
-  const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long edges = 0, verts = 0;
-  for (long long wd = (long long)blockIdx.x * (BS / kWave) + wid; wd < a.nwords; wd += nwv) {
+  for (long long wd = blockIdx.x; wd < a.nwords; wd += gridDim.x) {
     const long long v = wd * 64 + lane;
     const bool valid = v < a.n;
-    if (valid) {
+    if (valid && wid == 0) {
       const Mask<W> sd = load_mask<W>(a.seen, v);
       int pc = 0;
 #pragma unroll
@@ -502,7 +510,7 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
       verts += (unsigned long long)pc;
       edges += (unsigned long long)pc * (unsigned long long)(a.off[v + 1] - a.off[v]);
     }
-    for (int g = 0; g < W; ++g) {
+    for (int g = wid; g < W; g += NWV) {
       unsigned int *row = reinterpret_cast<unsigned int *>(d8 + lane * 68);
 #pragma unroll
       for (int j = 0; j < 16; ++j) row[j] = 0xFFFFFFFFu;
@@ -597,25 +605,26 @@ __global__ void k_binit_src(BArgs a, const int *roots, unsigned long long jw, un
   }
 }
 
-// Position-keyed checksum of each row (sum over i of mix(i, value)).
-__global__ void k_row_hash(const int *dev, long long cols, unsigned long long *out) {
-  const long long r = blockIdx.x;
+// Position-keyed checksum of each row (sum over i of mix(i, value), mod
+// 2^64). Block (r, c) sums chunk c of row r and adds it to out[r].
+__global__ void k_row_hash(const int *dev, long long cols, long long chunk, unsigned long long *out) {
+  const long long r = blockIdx.y;
+  const long long c0 = (long long)blockIdx.x * chunk;
+  const long long c1 = min(cols, c0 + chunk);
   unsigned long long h = 0;
-  for (long long i = threadIdx.x; i < cols; i += blockDim.x) {
+  for (long long i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
     unsigned long long z = ((unsigned long long)i << 32) ^ (unsigned int)dev[r * cols + i];
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     h += z ^ (z >> 31);
   }
-  __shared__ unsigned long long red[256];
-  red[threadIdx.x] = h;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+  __shared__ unsigned long long red[4];
+  if (lane_id() == 0) red[threadIdx.x / kWave] = h;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[r] = red[0];
+  if (threadIdx.x == 0) atomicAdd(&out[r], red[0] + red[1] + red[2] + red[3]);
 }
 
 struct BatchWorkspace {
@@ -757,7 +766,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
                             g->stream));
   }
   const void *kl = blevel_for(W);
-  const int lgrid = g->cus * std::max(1, c.blocks_per_cu > 0 ? c.blocks_per_cu : 8);
+  const int lgrid = c.grid_blocks > 0 ? c.grid_blocks : g->cus * std::max(1, c.blocks_per_cu > 0 ? c.blocks_per_cu : 8);
   long long i = 0;
   int kk = std::max(2, b->expect_launches);
   const long long max_launches = 4LL * (b->lcap + 4);
@@ -905,15 +914,19 @@ int ent_device_free(ent_graph *g, void *p) {
 
 int ent_device_row_hash(ent_graph *g, const int32_t *dev, int64_t rows, int64_t cols, uint64_t *hashes_host) {
   clear_error();
-  if (!g || !dev || !hashes_host || rows < 1 || cols < 1 || rows > (1 << 20))
+  if (!g || !dev || !hashes_host || rows < 1 || cols < 1 || rows > 65535)
     return set_error(ENT_EINVAL, "bad argument");
   std::lock_guard<std::mutex> lk(g->mu);
   HIP_TRY(hipSetDevice(g->device));
   unsigned long long *d = nullptr;
   HIP_TRY(hipMalloc(&d, sizeof(unsigned long long) * size_t(rows)));
-  void *args[] = {const_cast<int32_t **>(&dev), &cols, &d};
-  hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(&k_row_hash), dim3(unsigned(rows)), dim3(256), args, 0,
-                                 g->stream);
+  long long chunk = 1 << 16;
+  const long long nchunks = (cols + chunk - 1) / chunk;
+  void *args[] = {const_cast<int32_t **>(&dev), &cols, &chunk, &d};
+  hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long) * size_t(rows), g->stream);
+  if (e == hipSuccess)
+    e = hipLaunchKernel(reinterpret_cast<const void *>(&k_row_hash), dim3(unsigned(nchunks), unsigned(rows)), dim3(256),
+                        args, 0, g->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(hashes_host, d, sizeof(unsigned long long) * size_t(rows), hipMemcpyDeviceToHost, g->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g->stream);

@@ -893,6 +893,7 @@ int check_config(const ent_bfs_config *c) {
   if (c->bu_probe < 0 || c->bu_probe > 1 << 20) return set_error(ENT_EINVAL, "bu_probe %d", c->bu_probe);
   if (c->bu_chunk_words < 1 || c->bu_chunk_words > 64) return set_error(ENT_EINVAL, "bu_chunk_words %d", c->bu_chunk_words);
   if (c->graph_levels < 1 || c->graph_levels > 256) return set_error(ENT_EINVAL, "graph_levels %d", c->graph_levels);
+  if (c->grid_blocks < 0 || c->grid_blocks > (1 << 20)) return set_error(ENT_EINVAL, "grid_blocks %d", c->grid_blocks);
   return ENT_OK;
 }
 
@@ -912,7 +913,9 @@ Args make_args(ent_graph *g, int *depth) {
   a.qbuf[0] = g->qbuf[0];
   a.qbuf[1] = g->qbuf[1];
   a.qcap_s = g->qcap_s;
-  a.q_small = int(std::min<int64_t>(std::max<int64_t>(4096, g->n / 256), int64_t(kStripes) * g->qcap_s / 2));
+  if (g->cfg.queue_small < 0) a.q_small = 0;
+  else if (g->cfg.queue_small > 0) a.q_small = int(std::min<int64_t>(g->cfg.queue_small, int64_t(kStripes) * g->qcap_s / 2));
+  else a.q_small = int(std::min<int64_t>(std::max<int64_t>(4096, g->n / 256), int64_t(kStripes) * g->qcap_s / 2));
   a.ctl = g->ctl;
   a.m = g->m;
   a.n_active = g->n_active;
@@ -956,6 +959,7 @@ KernelSet kernels_for(int bs) {
 }
 
 int level_grid(ent_graph *g) {
+  if (g->cfg.grid_blocks > 0) return g->cfg.grid_blocks;
   const int bpc = g->cfg.blocks_per_cu > 0 ? g->cfg.blocks_per_cu : std::max(1, 2048 / g->cfg.block_size);
   return g->cus * bpc;
 }
@@ -1028,6 +1032,7 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
       if (per_cu < 1) return set_error(ENT_EDEVICE, "persistent kernel cannot be resident");
       void *args[] = {&a};
       g->last_grid = g->cus * per_cu;
+      if (cfg.grid_blocks > 0) g->last_grid = std::min(g->last_grid, cfg.grid_blocks);
       HIP_TRY(hipLaunchCooperativeKernel(ks.persist, dim3(g->last_grid), dim3(bs), args, 0, g->stream));
       HIP_TRY(hipMemcpyAsync(g->h_ctl, g->ctl, sizeof(State) * 2, hipMemcpyDeviceToHost, g->stream));
       HIP_TRY(hipStreamSynchronize(g->stream));
@@ -1167,7 +1172,7 @@ void ent_bfs_default_config(ent_bfs_config *c) {
   c->bu_probe = 16;
   c->bu_chunk_words = 8;
   c->graph_levels = 8;
-  c->hub_cache = 0;
+  c->queue_small = 0;
   c->batch_alpha = 4;
 }
 

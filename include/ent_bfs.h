@@ -67,11 +67,15 @@ typedef struct ent_bfs_config {
   int32_t bu_chunk_words;  /* 64-vertex bitmap words per wavefront chunk in
                               the bottom-up sweep: 1..64 */
   int32_t graph_levels;    /* phases per captured graph / queued batch */
-  int32_t hub_cache;       /* reserved (0) */
+  int32_t queue_small;     /* largest frontier expanded top-down from the
+                              frontier queue in one launch (0 = auto,
+                              negative = never) */
   int32_t batch_alpha;     /* multi-source search: bottom-up when the joint
                               frontier's degree sum * batch_alpha exceeds the
                               degree sum of vertices still missing sources */
-  int32_t reserved;
+  int32_t grid_blocks;     /* blocks per launch (0 = blocks_per_cu * CUs);
+                              the persistent form never exceeds its
+                              residency cap */
 } ent_bfs_config;
 
 typedef struct ent_stats {
