@@ -87,7 +87,7 @@ def load_tuned(dg, name):
 
 
 def bench_plain(args):
-    from enterprise_amd import DeviceGraph, HostGraph
+    from enterprise_amd import DeviceGraph, HostGraph, traffic
     from enterprise_amd import _lib as L
 
     scale = args.scale
@@ -157,7 +157,7 @@ def bench_plain(args):
                    "launch": dg.config.launch, "direction": dg.config.direction},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": None,
+                     "traffic": traffic.figure(f"plain_rmat{scale}"),
                      "kernel": "whole search (init + level kernels) per BFS, HIP events on "
                                "the handle's stream",
                      "alg_bytes_per_bfs_mean": int(np.mean(alg_bytes))},
@@ -194,6 +194,10 @@ def main():
     ap.add_argument("--rates", default="100,200,500,1000,2000,5000")
     ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--grid-side", type=int, default=4096)
+    ap.add_argument("--traffic", default=None, metavar="PATH",
+                    help="measure HBM bytes per search of --mode (plain or batch) with two "
+                         "rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by a copy kernel; "
+                         "writes the summary to PATH")
     ap.add_argument("--set", action="append", default=[],
                     help="config override key=value (ent_bfs_config field), repeatable")
     args = ap.parse_args()
@@ -202,7 +206,21 @@ def main():
               f"(batched and serving work shard by replicating the graph, see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
-    if args.mode == "plain":
+    if args.traffic:
+        from enterprise_amd import traffic
+        if args.mode == "plain":
+            child = ["--no-cpu", "--steps", str(args.steps), "--warmup", "0", "--check-roots", "0",
+                     "--scale", str(args.scale)]
+        elif args.mode == "batch":
+            child = ["--mode", "batch", "--steps", "1", "--warmup", "0"]
+        else:
+            raise SystemExit("--traffic supports --mode plain and batch")
+        child += [x for kv in args.set for x in ("--set", kv)]
+        out = traffic.measure(args.mode, child, os.path.join(os.path.dirname(os.path.abspath(args.traffic)),
+                                                             "traffic_runs"))
+        with open(args.traffic, "w") as f:
+            json.dump(out, f, indent=1)
+    elif args.mode == "plain":
         out = bench_plain(args)
     else:
         from enterprise_amd import benchmarks

@@ -39,7 +39,7 @@ class BfsConfig(C.Structure):
     _fields_ = [(name, C.c_int32) for name in (
         "direction", "launch", "alpha", "beta", "t_wave", "t_block", "t_grid",
         "piece", "block_size", "blocks_per_cu", "bu_probe", "bu_chunk_words",
-        "graph_levels", "queue_small", "batch_alpha", "grid_blocks")]
+        "graph_levels", "queue_small", "batch_alpha", "grid_blocks", "hub_cache", "hub_td")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

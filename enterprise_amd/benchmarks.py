@@ -41,6 +41,7 @@ def batch(args):
 
     import numpy as np
 
+    from . import traffic
     from .bfs import DeviceGraph
     from .graph import HostGraph
 
@@ -75,7 +76,8 @@ def batch(args):
     out = dg.device_alloc(k * g.n * 4)
     try:
         dg.batch_reserve(k, 48)
-        dg.bfs_batch_device(roots[:64], out)            # warm-up
+        if args.warmup > 0:
+            dg.bfs_batch_device(roots[:64], out)        # warm-up
         stats = []
         for _ in range(max(1, args.steps if args.steps < 64 else 3)):
             stats.append(dg.bfs_batch_device(roots, out))
@@ -99,7 +101,8 @@ def batch(args):
         "speedup_vs_plain": round(batch_gteps / plain_agg, 2),
         "rows_mismatching_plain": mism, "levels": best["levels"], "phases": best["phases"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / 8000.0, 4), "alg_bytes": best["alg_bytes"]},
+                     "frac": round(achieved / 8000.0, 4), "alg_bytes": best["alg_bytes"],
+                     "traffic": traffic.figure(f"batch_rmat{scale}")},
         "runs_ms": [round(s["host_ms"], 3) for s in stats],
     }
 

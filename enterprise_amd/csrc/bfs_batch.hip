@@ -391,6 +391,51 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   wave_counts<BS, W>(sm, jw, mf, alg, munf, act, 0, 0);
 }
 
+// Bottom-up tail for the vertex of this lane. x = new bits (a subset of tgt,
+// the active sources that had not seen v), so the vertex stays unfinished
+// for some active source iff tgt has bits outside x. The seen mask is read
+// back only for vertices that gained bits. act is wave-uniform: it is merged
+// with a wave OR only when some lane holds a source not yet in it.
+template <int W>
+__device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, long long v, const Mask<W> &tgt,
+                                         const Mask<W> &x, long long deg, unsigned long long &jw,
+                                         unsigned long long &mf, unsigned long long &alg, unsigned long long &munf,
+                                         Mask<W> &act, bool &any) {
+  int nzw = 0;
+  bool unf = false, fresh = false;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    nzw += x.w[w] != 0;
+    unf |= (tgt.w[w] & ~x.w[w]) != 0;
+    fresh |= (x.w[w] & ~act.w[w]) != 0;
+  }
+  any = nzw != 0;
+  if (__ballot(fresh)) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) act.w[w] |= wave_or64(x.w[w]);
+  }
+  if (!valid) return;
+  if (any) {
+    Mask<W> ns = load_mask<W>(a.seen, v);
+#pragma unroll
+    for (int w = 0; w < W; ++w) ns.w[w] |= x.w[w];
+    store_mask<W>(a.seen, v, ns);
+    jw += (unsigned long long)nzw;
+    mf += (unsigned long long)deg;
+    alg += (unsigned long long)nzw * (8ull + 12ull * (unsigned long long)deg);
+  }
+  if (unf) munf += (unsigned long long)deg;
+  Mask<W> z;
+#pragma unroll
+  for (int w = 0; w < W; ++w) z.w[w] = 0;
+  store_mask<W>(level_ptr(a, L + 2), v, z);
+}
+
+// Bottom-up (pull) level: every vertex v gathers the level-L masks of its
+// neighbours (only those in the joint frontier) until the OR covers
+// tgt = active & ~seen(v) or the list ends. Lanes probe their own first
+// `probe` neighbours two at a time; longer remaining lists are scanned by the
+// whole wavefront, one vertex at a time, with a wave OR per 64 neighbours.
 template <int BS, int W>
 __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const unsigned long long *fs = level_ptr(a, L);
@@ -406,13 +451,13 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   for (long long wd = (long long)blockIdx.x * (BS / kWave) + threadIdx.x / kWave; wd < a.nwords; wd += nwv) {
     const long long v = wd * 64 + lane;
     const bool valid = v < a.n;
-    Mask<W> sd, tgt, acc;
+    Mask<W> tgt, acc;
     bool need = false;
     long long s = 0, e = 0, p = 0;
 #pragma unroll
-    for (int w = 0; w < W; ++w) sd.w[w] = tgt.w[w] = acc.w[w] = 0;
+    for (int w = 0; w < W; ++w) tgt.w[w] = acc.w[w] = 0;
     if (valid) {
-      sd = load_mask<W>(a.seen, v);
+      const Mask<W> sd = load_mask<W>(a.seen, v);
       s = a.off[v];
       e = a.off[v + 1];
 #pragma unroll
@@ -425,34 +470,45 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     bool complete = !need;
     if (need) {
       const long long pe = min(e, s + a.probe);
-      for (p = s; p < pe; ++p) {
-        const int u = a.adj[p];
-        if (!bitv(jf, u)) continue;  // no source has u in its level-L frontier
-        const Mask<W> f = load_mask<W>(fs, u);
+      // Two neighbours per step: both joint-frontier tests, then both mask
+      // gathers are in flight before the OR and the early-exit test.
+      for (p = s; p < pe; p += 2) {
+        const int u0 = a.adj[p];
+        const bool two = p + 1 < pe;
+        const int u1 = two ? a.adj[p + 1] : u0;
+        const bool in0 = bitv(jf, u0), in1 = two && bitv(jf, u1);
+        Mask<W> f0, f1;
+#pragma unroll
+        for (int w = 0; w < W; ++w) f0.w[w] = f1.w[w] = 0;
+        if (in0) f0 = load_mask<W>(fs, u0);
+        if (in1) f1 = load_mask<W>(fs, u1);
         bool all = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-          acc.w[w] |= f.w[w];
+          acc.w[w] |= f0.w[w] | f1.w[w];
           all &= (acc.w[w] & tgt.w[w]) == tgt.w[w];
         }
         if (all) {
           complete = true;
+          p += 2;
           break;
         }
       }
+      if (p > pe) p = pe;
     }
-    // Long lists: the whole wavefront scans one vertex at a time, OR-reducing
-    // the masks each step and stopping once the targets are covered.
+    // Long lists: the whole wavefront scans one vertex at a time; the
+    // vertex's target and running OR are wave-uniform while it does.
     unsigned long long lmask = __ballot(need && !complete && p < e);
     while (lmask) {
       const int src = __ffsll((long long)lmask) - 1;
       lmask &= lmask - 1;
-      const long long pp = __shfl(p, src), ee = __shfl(e, src);
+      const long long pp = (long long)read_lane64((unsigned long long)p, src);
+      const long long ee = (long long)read_lane64((unsigned long long)e, src);
       Mask<W> t2, a2;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        t2.w[w] = __shfl(tgt.w[w], src);
-        a2.w[w] = __shfl(acc.w[w], src);
+        t2.w[w] = read_lane64(tgt.w[w], src);
+        a2.w[w] = read_lane64(acc.w[w], src);
       }
       for (long long q0 = pp; q0 < ee; q0 += kWave) {
         const long long q = q0 + lane;
@@ -466,10 +522,7 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
         bool all = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-          unsigned long long x = f.w[w];
-#pragma unroll
-          for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
-          a2.w[w] |= x;
+          a2.w[w] |= wave_or64(f.w[w]);
           all &= (a2.w[w] & t2.w[w]) == t2.w[w];
         }
         if (all) break;
@@ -478,10 +531,10 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     }
     Mask<W> x;
 #pragma unroll
-    for (int w = 0; w < W; ++w) x.w[w] = acc.w[w] & ~sd.w[w];
+    for (int w = 0; w < W; ++w) x.w[w] = acc.w[w] & tgt.w[w];
     if (valid) store_mask<W>(nx, v, x);
     bool any;
-    settle_vertex<W>(a, L, valid, v, sd, x, e - s, sm.S.active, jw, mf, alg, munf, act, any);
+    settle_bu<W>(a, L, valid, v, tgt, x, e - s, jw, mf, alg, munf, act, any);
     const unsigned long long bits = __ballot(any);
     if (lane == 0) jn[wd] = bits;
   }
@@ -541,20 +594,33 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
   wave_counts<BS, W>(sm, 0, 0, 0, 0, none, edges, verts);
 }
 
+// Launch i runs as two kernels queued back to back: the bottom-up phase has
+// its own kernel (its mask registers would otherwise cap the occupancy of
+// every phase), the other phases share one. Both derive the same state for
+// launch i; the one whose phase it is does the work.
 template <int BS, int W>
 __global__ __launch_bounds__(BS) void k_blevel(BArgs a, long long i) {
   __shared__ BSmem<BS> sm;
   begin_blaunch<BS, W>(a, sm, i);
   const BState &S = sm.S;
-  if (S.done) return;
+  if (S.done || S.phase == BPH_BU) return;
   const int slot = int(i % 3);
   switch (S.phase) {
     case BPH_TD: btd_phase<BS, W>(a, sm, S.level); break;
     case BPH_APPLY: bapply_phase<BS, W>(a, sm, S.level); break;
-    case BPH_BU: bbu_phase<BS, W>(a, sm, S.level); break;
     default: boutput_phase<BS, W>(a, sm, S.levels); break;
   }
   if (S.phase != BPH_TD) block_counts_flush(a, sm, slot);
+}
+
+template <int BS, int W>
+__global__ __launch_bounds__(BS) void k_blevel_bu(BArgs a, long long i) {
+  __shared__ BSmem<BS> sm;
+  begin_blaunch<BS, W>(a, sm, i);
+  const BState &S = sm.S;
+  if (S.done || S.phase != BPH_BU) return;
+  bbu_phase<BS, W>(a, sm, S.level);
+  block_counts_flush(a, sm, int(i % 3));
 }
 
 // Zeroes seen, lv[0], lv[1] and the joint frontier; clears counters; sets the
@@ -693,17 +759,22 @@ int reserve(ent_graph *g, int k, int lcap) {
   return ENT_OK;
 }
 
+struct BKernels {
+  const void *gen, *bu;
+};
+
 template <int W>
-const void *blevel_kernel() {
-  return reinterpret_cast<const void *>(&k_blevel<kBatchBS, W>);
+BKernels blevel_kernels() {
+  return {reinterpret_cast<const void *>(&k_blevel<kBatchBS, W>),
+          reinterpret_cast<const void *>(&k_blevel_bu<kBatchBS, W>)};
 }
 
-const void *blevel_for(int W) {
+BKernels blevel_for(int W) {
   switch (W) {
-    case 1: return blevel_kernel<1>();
-    case 2: return blevel_kernel<2>();
-    case 4: return blevel_kernel<4>();
-    default: return blevel_kernel<8>();
+    case 1: return blevel_kernels<1>();
+    case 2: return blevel_kernels<2>();
+    case 4: return blevel_kernels<4>();
+    default: return blevel_kernels<8>();
   }
 }
 
@@ -765,7 +836,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
     HIP_TRY(hipLaunchKernel(reinterpret_cast<const void *>(&k_binit_src), dim3((k + 255) / 256), dim3(256), args, 0,
                             g->stream));
   }
-  const void *kl = blevel_for(W);
+  const BKernels kl = blevel_for(W);
   const int lgrid = c.grid_blocks > 0 ? c.grid_blocks : g->cus * std::max(1, c.blocks_per_cu > 0 ? c.blocks_per_cu : 8);
   long long i = 0;
   int kk = std::max(2, b->expect_launches);
@@ -773,7 +844,8 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   for (;;) {
     for (int j = 0; j < kk; ++j, ++i) {
       void *args[] = {&a, &i};
-      HIP_TRY(hipLaunchKernel(kl, dim3(lgrid), dim3(kBatchBS), args, 0, g->stream));
+      HIP_TRY(hipLaunchKernel(kl.gen, dim3(lgrid), dim3(kBatchBS), args, 0, g->stream));
+      HIP_TRY(hipLaunchKernel(kl.bu, dim3(lgrid), dim3(kBatchBS), args, 0, g->stream));
     }
     HIP_TRY(hipMemcpyAsync(b->h_ctl, b->ctl, sizeof(BState) * 2, hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));

@@ -126,7 +126,19 @@ struct Args {
   int chunk_words;
   int qcap_s;   // entries per queue stripe
   int q_small;  // largest frontier expanded from the queue
+  // LDS hub cache (hub_mask = entries - 1, or -1 when off): hub_slot[s] is
+  // the highest-degree vertex v with v & hub_mask == s (or -1), hub_tag[s]
+  // the level at which it joined the frontier (-1: not yet this search).
+  const int *hub_slot;
+  int *hub_tag;
+  int hub_mask;
+  int hub_min;  // smallest degree in the table
+  int hub_td;   // top-down deferred hub marks
 };
+
+// Dynamic LDS: the hub table (int32 per slot) and, for top-down, one flag
+// byte per slot after it.
+extern __shared__ int hub_lds[];
 
 template <int BS>
 struct Smem {
@@ -232,8 +244,25 @@ __device__ __forceinline__ long long xcd_wave_id() {
   return eb * (blockDim.x / kWave) + threadIdx.x / kWave;
 }
 
+// Records that v (degree d) joined the frontier of `level` if it is its
+// slot's hub.
+__device__ __forceinline__ void hub_note(const Args &a, int v, long long d, int level) {
+  if (a.hub_mask >= 0 && d >= a.hub_min) {
+    const int s = v & a.hub_mask;
+    if (a.hub_slot[s] == v) a.hub_tag[s] = level;
+  }
+}
+
 __device__ __forceinline__ void td_mark(const Args &a, int u) {
   if (!bit_of(a.visited, u)) a.mark[u] = 1;
+}
+
+// Top-down edge with the hub table: a hub neighbour only sets its slot's LDS
+// flag; the block marks each flagged, unvisited hub once at the end.
+__device__ __forceinline__ void td_mark_hub(const Args &a, int u, unsigned char *flags) {
+  const int s = u & a.hub_mask;
+  if (hub_lds[s] == u) flags[s] = 1;
+  else if (!bit_of(a.visited, u)) a.mark[u] = 1;
 }
 
 // Marks the unvisited neighbours adj[i] for i = s + t, s + t + stride, ...
@@ -252,24 +281,50 @@ __device__ __forceinline__ void td_mark_range(const Args &a, long long s, long l
   for (; i < e; i += stride) td_mark(a, a.adj[i]);
 }
 
+__device__ __forceinline__ void td_mark_range_hub(const Args &a, long long s, long long e, int t, int stride,
+                                                  unsigned char *flags) {
+  long long i = s + t;
+  for (; i + 3LL * stride < e; i += 4LL * stride) {
+    const int u0 = a.adj[i], u1 = a.adj[i + stride], u2 = a.adj[i + 2 * stride], u3 = a.adj[i + 3 * stride];
+    td_mark_hub(a, u0, flags);
+    td_mark_hub(a, u1, flags);
+    td_mark_hub(a, u2, flags);
+    td_mark_hub(a, u3, flags);
+  }
+  for (; i < e; i += stride) td_mark_hub(a, a.adj[i], flags);
+}
+
 // Top-down level L. Grid-class pieces and block-class hubs come from the
 // static lists (skipped unless in frontier L); thread- and wavefront-class
 // frontier vertices come from a compacting sweep of front[L & 1]. Every
 // unvisited neighbour gets a byte mark; nothing is counted here.
-template <int BS>
+template <int BS, bool HUB>
 __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
   const unsigned long long *fr = a.front[L & 1];
+  const int H = a.hub_mask + 1;
+  unsigned char *flags = reinterpret_cast<unsigned char *>(hub_lds + H);
+  if (HUB) {
+    for (int s = threadIdx.x; s < H; s += BS) {
+      hub_lds[s] = a.hub_slot[s];
+      flags[s] = 0;
+    }
+    __syncthreads();
+  }
+  auto mark_range = [&](long long s, long long e, int t, int stride) {
+    if (HUB) td_mark_range_hub(a, s, e, t, stride, flags);
+    else td_mark_range(a, s, e, t, stride);
+  };
   for (int p = blockIdx.x; p < a.n_pieces; p += gridDim.x) {
     const int v = a.piece_v[p];
     if (!bit_of(fr, v)) continue;
     const long long s = a.off[v] + (long long)a.piece_k[p] * a.piece;
     const long long e = min(a.off[v + 1], s + a.piece);
-    td_mark_range(a, s, e, threadIdx.x, BS);
+    mark_range(s, e, threadIdx.x, BS);
   }
   for (int h = blockIdx.x; h < a.n_hub_block; h += gridDim.x) {
     const int v = a.hub_block[h];
     if (!bit_of(fr, v)) continue;
-    td_mark_range(a, a.off[v], a.off[v + 1], threadIdx.x, BS);
+    mark_range(a.off[v], a.off[v + 1], threadIdx.x, BS);
   }
   const int wid = threadIdx.x / kWave, lane = lane_id();
   unsigned long long *wc = sm.cand[wid];
@@ -291,7 +346,7 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
         const int v = int((ch * CW + j) * 64 + bp);
         s = a.off[v];
         e = a.off[v + 1];
-        if (e - s < a.t_wave) td_mark_range(a, s, e, 0, 1);
+        if (e - s < a.t_wave) mark_range(s, e, 0, 1);
       }
       const long long d = e - s;
       unsigned long long wm = __ballot(k < total && d >= a.t_wave && d < a.t_block);
@@ -299,10 +354,17 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
         const int src = __ffsll((long long)wm) - 1;
         wm &= wm - 1;
         const long long ss = __shfl(s, src), ee = __shfl(e, src);
-        td_mark_range(a, ss, ee, lane, kWave);
+        mark_range(ss, ee, lane, kWave);
       }
     }
     wave_sync();
+  }
+  if (HUB) {
+    __syncthreads();
+    for (int s = threadIdx.x; s < H; s += BS) {
+      const int v = hub_lds[s];
+      if (flags[s] && !bit_of(a.visited, v)) a.mark[v] = 1;
+    }
   }
 }
 
@@ -331,6 +393,7 @@ __device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         nf += 1;
         mf += (unsigned long long)d;
         stage_push(sm, int(v), d);
+        hub_note(a, int(v), d, L + 1);
       }
       if (lane == 0 && nw) a.visited[w] = vis | nw;
     }
@@ -347,9 +410,14 @@ __device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
 // bu_probe neighbours, hands longer unresolved lists to the whole wavefront
 // (ballot early exit), then writes its words of the next frontier and of the
 // visited bitmap without atomics, the depths, and the counts.
-template <int BS>
+template <int BS, bool HUB>
 __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
   const unsigned long long *fr = a.front[L & 1];
+  const int hm = a.hub_mask;
+  if (HUB) {  // the table's hubs that are in frontier L, -1 elsewhere
+    for (int s = threadIdx.x; s <= hm; s += BS) hub_lds[s] = a.hub_tag[s] == L ? a.hub_slot[s] : -1;
+    __syncthreads();
+  }
   unsigned long long *fn = a.front[(L + 1) & 1];
   const int wid = threadIdx.x / kWave, lane = lane_id();
   unsigned long long *wc = sm.cand[wid];
@@ -384,6 +452,12 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         // scan if any of them is in the frontier.
         for (p = s; p + 3 < pe; p += 4) {
           const int u0 = a.adj[p], u1 = a.adj[p + 1], u2 = a.adj[p + 2], u3 = a.adj[p + 3];
+          // A frontier hub among the four ends the scan from LDS.
+          if (HUB && (int(hub_lds[u0 & hm] == u0) | int(hub_lds[u1 & hm] == u1) | int(hub_lds[u2 & hm] == u2) |
+                      int(hub_lds[u3 & hm] == u3)) != 0) {
+            found = true;
+            break;
+          }
           // Non-short-circuit OR: all four loads are issued before the test.
           if ((int(bit_of(fr, u0)) | int(bit_of(fr, u1)) | int(bit_of(fr, u2)) | int(bit_of(fr, u3))) != 0) {
             found = true;
@@ -405,6 +479,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         nf += 1;
         mf += (unsigned long long)(e - s);
         stage_push(sm, v, e - s);
+        hub_note(a, v, e - s, L + 1);
       }
       unsigned long long lmask = __ballot(k < total && !found && p < e);
       while (lmask) {
@@ -419,7 +494,14 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         bool hit_any = false;
         for (long long q0 = pp; q0 < ee; q0 += 2 * kWave) {
           const long long q = q0 + lane, q2 = q + kWave;
-          const bool hit = (int(q < ee && bit_of(fr, a.adj[q])) | int(q2 < ee && bit_of(fr, a.adj[q2]))) != 0;
+          bool hit;
+          if (HUB) {
+            const int x0 = q < ee ? a.adj[q] : -1, x1 = q2 < ee ? a.adj[q2] : -1;
+            hit = (x0 >= 0 && (hub_lds[x0 & hm] == x0 || bit_of(fr, x0))) ||
+                  (x1 >= 0 && (hub_lds[x1 & hm] == x1 || bit_of(fr, x1)));
+          } else {
+            hit = (int(q < ee && bit_of(fr, a.adj[q])) | int(q2 < ee && bit_of(fr, a.adj[q2]))) != 0;
+          }
           if (__ballot(hit)) {
             hit_any = true;
             break;
@@ -431,6 +513,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
           nf += 1;
           mf += (unsigned long long)(ee - ss);
           stage_push(sm, vv, ee - ss);
+          hub_note(a, vv, ee - ss, L + 1);
         }
       }
     }
@@ -446,8 +529,8 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
   add_counts(a, sm, slot, (L + 1) & 1);
 }
 
-// Top-down level L from the striped queue (small frontiers without
-// block-class vertices): one launch per level. A lane (or, for degree >=
+// Top-down level L from the striped queue (small frontiers whose vertices all
+// have degree < 4 * t_wave): one launch per level. A lane (or, for degree >=
 // t_wave, the whole wavefront) expands a frontier vertex; each unvisited
 // neighbour is claimed with an atomic OR on its visited word, and the
 // claiming lane writes its depth and stages it for the next queue.
@@ -464,17 +547,35 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
   const int total = sm.qpre[kStripes];
   const int *qin = a.qbuf[L & 1];
   unsigned long long nf = 0, mf = 0;
-  auto visit = [&](int u) {
-    const unsigned long long bit = 1ull << (u & 63);
-    unsigned long long *w = a.visited + (u >> 6);
-    if (*w & bit) return;
-    const unsigned long long old = atomicOr(w, bit);
-    if (old & bit) return;
+  // Claims the unvisited among adj[i], i = i0, i0 + stride, ... < e, four
+  // edges per step: visited bits are tested first, then the atomics of the
+  // step are issued together so their return latencies overlap.
+  auto claim = [&](int u, unsigned long long old) {
+    if (old & (1ull << (u & 63))) return;
     a.depth[u] = L + 1;
     const long long d = a.off[u + 1] - a.off[u];
     nf += 1;
     mf += (unsigned long long)d;
     stage_push(sm, u, d);
+  };
+  auto visit_range = [&](long long i0, long long e, int stride) {
+    for (long long i = i0; i < e; i += 4LL * stride) {
+      int u[4];
+      bool cand[4];
+      unsigned long long old[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const long long ij = i + (long long)j * stride;
+        u[j] = ij < e ? a.adj[ij] : 0;
+        cand[j] = ij < e && !bit_of(a.visited, u[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        old[j] = cand[j] ? atomicOr(a.visited + (u[j] >> 6), 1ull << (u[j] & 63)) : ~0ull;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (cand[j]) claim(u[j], old[j]);
+    }
   };
   const long long gt = (long long)blockIdx.x * BS + threadIdx.x;
   const long long nt = (long long)gridDim.x * BS;
@@ -491,15 +592,14 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
       const int v = qin[(long long)lo * a.qcap_s + (k - sm.qpre[lo])];
       s = a.off[v];
       e = a.off[v + 1];
-      if (e - s < a.t_wave)
-        for (long long i = s; i < e; ++i) visit(a.adj[i]);
+      if (e - s < a.t_wave) visit_range(s, e, 1);
     }
     unsigned long long wm = __ballot(k < total && e - s >= a.t_wave);
     while (wm) {
       const int src = __ffsll((long long)wm) - 1;
       wm &= wm - 1;
       const long long ss = __shfl(s, src), ee = __shfl(e, src);
-      for (long long i = ss + lane; i < ee; i += kWave) visit(a.adj[i]);
+      visit_range(ss + lane, ee, kWave);
     }
   }
   wave_add_counts(sm, nf, mf);
@@ -515,8 +615,10 @@ __device__ void rebuild_phase(const Args &a, Smem<BS> &sm, int L) {
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   for (long long w = xcd_wave_id(); w < a.nwords; w += nwv) {
     const long long v = w * 64 + lane;
-    const unsigned long long word = __ballot(v < a.n && a.depth[v] == L);
+    const bool in = v < a.n && a.depth[v] == L;
+    const unsigned long long word = __ballot(in);
     if (lane == 0) fr[w] = word;
+    if (in && a.hub_mask >= 0) hub_note(a, int(v), a.off[v + 1] - a.off[v], L);
   }
 }
 
@@ -588,7 +690,8 @@ __device__ __forceinline__ void next_state(const Args &a, State &s, const unsign
   if (next_bu) {
     s.phase = bm ? PH_BU : PH_REBUILD;
     s.pending = PH_BU;
-  } else if (q && nf <= (unsigned long long)a.q_small && tot[K_MAXDEG] < (unsigned long long)a.t_block) {
+  } else if (q && nf <= (unsigned long long)a.q_small &&
+             tot[K_MAXDEG] < (unsigned long long)min(a.t_block, 4 * a.t_wave)) {
     s.phase = PH_TDQ;
   } else {
     s.phase = bm ? PH_TD : PH_REBUILD;
@@ -689,6 +792,7 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu) {
     c->cnt[i / kC].v[i % kC] = x;
   }
   if (gt == 0) a.qbuf[0][0] = root;
+  for (long long s = gt; s <= a.hub_mask; s += nt) a.hub_tag[s] = a.hub_slot[s] == root ? 0 : -1;
   for (long long i = gt; i < kStripes * 32; i += nt) c->arrive[i] = 0;
   if (gt == 0) {
     State S{};
@@ -706,8 +810,14 @@ __device__ __forceinline__ void run_phase(const Args &a, Smem<BS> &sm, long long
   const int L = sm.S.level;
   const int slot = int(i % 3);
   switch (sm.S.phase) {
-    case PH_TD: td_phase<BS>(a, sm, L); break;
-    case PH_BU: bu_phase<BS>(a, sm, L, slot); break;
+    case PH_TD:
+      if (a.hub_td && a.hub_mask >= 0) td_phase<BS, true>(a, sm, L);
+      else td_phase<BS, false>(a, sm, L);
+      break;
+    case PH_BU:
+      if (a.hub_mask >= 0) bu_phase<BS, true>(a, sm, L, slot);
+      else bu_phase<BS, false>(a, sm, L, slot);
+      break;
     case PH_INSPECT: inspect_phase<BS>(a, sm, L, slot); break;
     case PH_TDQ: tdq_phase<BS>(a, sm, L, slot); break;
     default: rebuild_phase<BS>(a, sm, L); break;
@@ -819,6 +929,8 @@ void release(ent_graph *g) {
   if (g->graph) ignore(hipGraphDestroy(g->graph));
   g->store.reset();
   ignore(hipFree(g->hub_block));
+  ignore(hipFree(g->hub_slot));
+  ignore(hipFree(g->hub_tag));
   ignore(hipFree(g->piece_v));
   ignore(hipFree(g->piece_k));
   ignore(hipFree(g->depth));
@@ -836,10 +948,56 @@ void release(ent_graph *g) {
   delete g;
 }
 
+// Builds the LDS hub table for hub_cache = log2 entries: slot s gets the
+// highest-degree vertex v (degree >= t_wave, smallest id on ties) with
+// v mod 2^hub_cache = s, or -1. hub_tag (same length) is cleared by k_init.
+int build_hub_table(ent_graph *g) {
+  const ent_bfs_config &c = g->cfg;
+  if (g->lists_hub == c.hub_cache && (c.hub_cache == 0 || g->lists_hub_tw == c.t_wave)) return ENT_OK;
+  dev_free(g, g->hub_slot, sizeof(int) * size_t(std::max(g->hub_slots, 1)));
+  dev_free(g, g->hub_tag, sizeof(int) * size_t(std::max(g->hub_slots, 1)));
+  g->hub_slot = g->hub_tag = nullptr;
+  g->hub_slots = 0;
+  g->hub_min = 0;
+  g->lists_hub = -1;
+  if (c.hub_cache > 0) {
+    const int H = 1 << c.hub_cache;
+    std::vector<int32_t> slot(size_t(H), -1);
+    for (int64_t v = 0; v < g->n; ++v) {
+      const int32_t d = g->store->h_deg[size_t(v)];
+      if (d < c.t_wave) continue;
+      int32_t &cur = slot[size_t(v & (H - 1))];
+      if (cur < 0 || d > g->store->h_deg[size_t(cur)]) cur = int32_t(v);
+    }
+    int32_t dmin = INT32_MAX;
+    for (int32_t v : slot)
+      if (v >= 0) dmin = std::min(dmin, g->store->h_deg[size_t(v)]);
+    int rc;
+    if ((rc = dev_alloc(g, reinterpret_cast<void **>(&g->hub_slot), sizeof(int) * size_t(H)))) return rc;
+    if ((rc = dev_alloc(g, reinterpret_cast<void **>(&g->hub_tag), sizeof(int) * size_t(H)))) return rc;
+    HIP_TRY(hipMemcpy(g->hub_slot, slot.data(), sizeof(int) * size_t(H), hipMemcpyHostToDevice));
+    g->hub_slots = H;
+    g->hub_min = dmin == INT32_MAX ? INT32_MAX : dmin;
+  }
+  g->lists_hub = c.hub_cache;
+  g->lists_hub_tw = c.t_wave;
+  return ENT_OK;
+}
+
+// Dynamic LDS bytes of the level kernels: the hub table, plus a flag byte per
+// slot when top-down defers hub marks.
+size_t hub_lds_bytes(const ent_bfs_config &c) {
+  if (c.hub_cache <= 0) return 0;
+  const size_t H = size_t(1) << c.hub_cache;
+  return H * sizeof(int) + (c.hub_td ? H : 0);
+}
+
 // Rebuilds the static block-class hub list and grid-class piece table for the
-// thresholds in g->cfg (degrees from the host copy).
+// thresholds in g->cfg (degrees from the host copy), and the LDS hub table.
 int build_lists(ent_graph *g) {
   const ent_bfs_config &c = g->cfg;
+  const int hrc = build_hub_table(g);
+  if (hrc) return hrc;
   if (g->lists_t_block == c.t_block && g->lists_t_grid == c.t_grid && g->lists_piece == c.piece) return ENT_OK;
   std::vector<int32_t> hubs, pv, pk;
   for (int64_t v = 0; v < g->n; ++v) {
@@ -894,6 +1052,9 @@ int check_config(const ent_bfs_config *c) {
   if (c->bu_chunk_words < 1 || c->bu_chunk_words > 64) return set_error(ENT_EINVAL, "bu_chunk_words %d", c->bu_chunk_words);
   if (c->graph_levels < 1 || c->graph_levels > 256) return set_error(ENT_EINVAL, "graph_levels %d", c->graph_levels);
   if (c->grid_blocks < 0 || c->grid_blocks > (1 << 20)) return set_error(ENT_EINVAL, "grid_blocks %d", c->grid_blocks);
+  if (c->hub_cache != 0 && (c->hub_cache < 6 || c->hub_cache > 13))
+    return set_error(ENT_EINVAL, "hub_cache %d not 0 or in [6, 13]", c->hub_cache);
+  if (c->hub_td != 0 && c->hub_td != 1) return set_error(ENT_EINVAL, "hub_td %d not 0 or 1", c->hub_td);
   return ENT_OK;
 }
 
@@ -932,6 +1093,11 @@ Args make_args(ent_graph *g, int *depth) {
   a.beta = c.beta;
   a.bu_probe = c.bu_probe;
   a.chunk_words = c.bu_chunk_words;
+  a.hub_slot = g->hub_slot;
+  a.hub_tag = g->hub_tag;
+  a.hub_mask = g->hub_slots > 0 ? g->hub_slots - 1 : -1;
+  a.hub_min = g->hub_min;
+  a.hub_td = c.hub_td;
   return a;
 }
 
@@ -972,11 +1138,12 @@ int level_grid(ent_graph *g) {
 int persistent_blocks_per_cu(ent_graph *g, const KernelSet &ks) {
   const int bs = g->cfg.block_size;
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ks.persist, bs, 0) != hipSuccess) {
+  const size_t dyn = hub_lds_bytes(g->cfg);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ks.persist, bs, dyn) != hipSuccess) {
     ignore(hipGetLastError());
     return 0;
   }
-  const size_t lds_block = (ks.smem + 2047) / 2048 * 2048;
+  const size_t lds_block = (ks.smem + dyn + 2047) / 2048 * 2048;
   const int lds_limit = int((160 * 1024) / lds_block);
   const int wave_limit = std::max(1, 16 / (bs / kWave));
   int per_cu = std::min(std::min(occ, lds_limit), std::min(wave_limit, 8));
@@ -984,8 +1151,8 @@ int persistent_blocks_per_cu(ent_graph *g, const KernelSet &ks) {
   return per_cu;
 }
 
-hipError_t launch(const void *fn, int grid, int bs, void **args, hipStream_t s) {
-  return hipLaunchKernel(fn, dim3(grid), dim3(bs), args, 0, s);
+hipError_t launch(const void *fn, int grid, int bs, void **args, hipStream_t s, size_t dyn = 0) {
+  return hipLaunchKernel(fn, dim3(grid), dim3(bs), args, dyn, s);
 }
 
 // Queues the per-phase kernel k times (used directly and under capture).
@@ -997,7 +1164,7 @@ hipError_t queue_phases(ent_graph *g, const KernelSet &ks, Args *a, long long i0
   for (int j = 0; j < k; ++j) {
     long long i = i0 + j;
     void *args[] = {a, &i};
-    hipError_t e = launch(ks.level, grid, g->cfg.block_size, args, g->stream);
+    hipError_t e = launch(ks.level, grid, g->cfg.block_size, args, g->stream, hub_lds_bytes(g->cfg));
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -1017,6 +1184,9 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
   int rc = build_lists(g);
   if (rc) return rc;
   const KernelSet ks = kernels_for(cfg.block_size);
+  if (ks.smem + hub_lds_bytes(cfg) > 65536)
+    return set_error(ENT_EINVAL, "hub_cache %d with block_size %d needs %zu bytes of LDS per block (64 KiB max)",
+                     cfg.hub_cache, cfg.block_size, ks.smem + hub_lds_bytes(cfg));
   Args a = make_args(g, depth_dev);
   const int bs = cfg.block_size;
   int first_bu = cfg.direction == ENT_DIR_BOTTOMUP ? 1 : 0;
@@ -1033,7 +1203,8 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
       void *args[] = {&a};
       g->last_grid = g->cus * per_cu;
       if (cfg.grid_blocks > 0) g->last_grid = std::min(g->last_grid, cfg.grid_blocks);
-      HIP_TRY(hipLaunchCooperativeKernel(ks.persist, dim3(g->last_grid), dim3(bs), args, 0, g->stream));
+      HIP_TRY(hipLaunchCooperativeKernel(ks.persist, dim3(g->last_grid), dim3(bs), args,
+                                         unsigned(hub_lds_bytes(cfg)), g->stream));
       HIP_TRY(hipMemcpyAsync(g->h_ctl, g->ctl, sizeof(State) * 2, hipMemcpyDeviceToHost, g->stream));
       HIP_TRY(hipStreamSynchronize(g->stream));
       const State &s0 = g->h_ctl->st[0], &s1 = g->h_ctl->st[1];
@@ -1174,6 +1345,8 @@ void ent_bfs_default_config(ent_bfs_config *c) {
   c->graph_levels = 8;
   c->queue_small = 0;
   c->batch_alpha = 4;
+  c->hub_cache = 0;
+  c->hub_td = 0;
 }
 
 int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int64_t m, int device,

@@ -30,6 +30,29 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
   return x;
 }
 
+// Bitwise OR over all 64 lanes with the same DPP pattern, returned as a
+// wave-uniform value (read from lane 63). All lanes must be active.
+__device__ __forceinline__ uint32_t wave_or32(uint32_t v) {
+  int x = int(v);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+  return uint32_t(__builtin_amdgcn_readlane(x, 63));
+}
+
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+  return uint64_t(wave_or32(uint32_t(v))) | (uint64_t(wave_or32(uint32_t(v >> 32))) << 32);
+}
+
+// Value of a 64-bit register in lane src (src wave-uniform), as a scalar.
+__device__ __forceinline__ uint64_t read_lane64(uint64_t v, int src) {
+  return uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v)), src))) |
+         (uint64_t(uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v >> 32)), src))) << 32);
+}
+
 __device__ __forceinline__ int wave_read(int x, int lane) {
   return __builtin_amdgcn_readlane(x, lane);
 }

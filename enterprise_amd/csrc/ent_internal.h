@@ -51,6 +51,11 @@ struct ent_graph {
   int *piece_k = nullptr;
   int n_hub_block = 0, n_pieces = 0;
   int lists_t_block = -1, lists_t_grid = -1, lists_piece = -1;
+  int *hub_slot = nullptr;  // LDS hub table source (hub_slots entries)
+  int *hub_tag = nullptr;   // level at which hub_slot[s] joined the frontier
+  int hub_slots = 0;        // entries built (0: none)
+  int hub_min = 0;          // smallest degree in the table
+  int lists_hub = -1, lists_hub_tw = -1;
   int *depth = nullptr;
   unsigned long long *visited = nullptr;
   unsigned long long *front[2] = {nullptr, nullptr};

@@ -76,6 +76,15 @@ typedef struct ent_bfs_config {
   int32_t grid_blocks;     /* blocks per launch (0 = blocks_per_cu * CUs);
                               the persistent form never exceeds its
                               residency cap */
+  int32_t hub_cache;       /* LDS hub table of 2^hub_cache entries (6..13;
+                              0 = off): slot s holds the highest-degree vertex
+                              v (degree >= t_wave) with v mod 2^hub_cache = s.
+                              Bottom-up tests neighbours against the table's
+                              frontier hubs before the frontier bitmap
+                              (comm.h:172-177, expander.cuh:454-494) */
+  int32_t hub_td;          /* with hub_cache: top-down flags hub neighbours in
+                              LDS and marks each flagged hub once per block
+                              (expander.cuh:96-109, :172-185) */
 } ent_bfs_config;
 
 typedef struct ent_stats {

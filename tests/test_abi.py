@@ -44,6 +44,8 @@ def test_default_config():
     c = default_config()
     assert c.block_size in (64, 128, 256, 512, 1024)
     assert 1 <= c.t_wave <= c.t_block <= c.t_grid and c.piece <= c.t_grid
+    assert c.hub_cache == 0 or 6 <= c.hub_cache <= 13
+    assert c.hub_td in (0, 1)
 
 
 def test_upload_rejects_malformed_csr():

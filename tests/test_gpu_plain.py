@@ -139,3 +139,62 @@ def test_many_roots_schedule_independent(oracle, rmat16):
                 assert np.array_equal(got, want)
     finally:
         dg.set_config(base)
+
+
+HUBS = [(6, 0), (10, 1), (13, 1), (13, 0)]
+
+
+@pytest.mark.parametrize("launch", list(LAUNCHES))
+@pytest.mark.parametrize("hub", HUBS, ids=[f"h{h}td{t}" for h, t in HUBS])
+def test_hub_cache(oracle, rmat16, launch, hub):
+    """LDS hub table: frontier-hub membership in bottom-up and deferred hub
+    marks in top-down leave depths and counts bit-exact, with the default and
+    the tiny class thresholds, under every direction policy."""
+    g, dg = rmat16
+    base = dg.config
+    h, td = hub
+    try:
+        for classes in ({}, TINY_CLASSES):
+            for dname, d in DIRS.items():
+                dg.configure(direction=d, launch=LAUNCHES[launch], hub_cache=h, hub_td=td, **classes)
+                for r in g.pick_roots(2, 91):
+                    _check(oracle, g, dg, int(r), f"rmat16/hub{h}td{td}/{dname}/{launch}/{len(classes)}")
+    finally:
+        dg.set_config(base)
+
+
+def test_hub_cache_star_and_closed_forms(oracle):
+    """Hub roots and hub-only frontiers: a 70,000-leaf star (the centre is the
+    only hub), and closed forms whose degrees fall below t_wave (empty table)."""
+    g, _ = graphs.star(70000)
+    with DeviceGraph(g) as dg:
+        for launch in LAUNCHES.values():
+            for d in DIRS.values():
+                dg.configure(direction=d, launch=launch, hub_cache=8, hub_td=1)
+                for r in (0, 1, 69999):
+                    _check(oracle, g, dg, r, f"star70000/hub/{d}/{launch}")
+    for name in ("hypercube10", "grid37x23", "disjoint", "complete40", "bintree11"):
+        g, depth_fn = graphs.CLOSED_FORMS[name]()
+        with DeviceGraph(g) as dg:
+            for d in DIRS.values():
+                dg.configure(direction=d, hub_cache=7, hub_td=1, t_wave=2)
+                for r in sorted({0, g.n // 2, g.n - 1}):
+                    st = _check(oracle, g, dg, r, f"{name}/hub/{d}")
+                    got, _ = dg.bfs(r)
+                    assert np.array_equal(got, np.asarray(depth_fn(r), dtype=np.int32)), (name, r, st)
+
+
+def test_hub_cache_rejects_bad_sizes(oracle, rmat16):
+    g, dg = rmat16
+    base = dg.config
+    try:
+        for bad in (1, 5, 14):
+            with pytest.raises(L.EntError):
+                dg.configure(hub_cache=bad)
+        with pytest.raises(L.EntError):
+            dg.configure(hub_cache=10, hub_td=2)
+        # The largest table with deferred marks fits the largest block.
+        dg.configure(hub_cache=13, hub_td=1, block_size=1024)
+        _check(oracle, g, dg, int(g.pick_roots(1, 3)[0]), "rmat16/hub13td1/bs1024")
+    finally:
+        dg.set_config(base)
