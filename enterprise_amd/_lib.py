@@ -39,7 +39,7 @@ class BfsConfig(C.Structure):
     _fields_ = [(name, C.c_int32) for name in (
         "direction", "launch", "alpha", "beta", "t_wave", "t_block", "t_grid",
         "piece", "block_size", "blocks_per_cu", "bu_probe", "bu_chunk_words",
-        "graph_levels", "queue_small", "batch_alpha", "grid_blocks", "hub_cache", "hub_td")]
+        "graph_levels", "queue_small", "batch_alpha", "grid_blocks", "hub_cache", "hub_td", "queue_max_degree")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -124,13 +124,15 @@ class EntError(RuntimeError):
 
 
 def lib():
-    """Loads libent_bfs.so once; raises if it has not been built."""
+    """Loads libent_bfs.so once (the in-tree build, or the file named by the
+    ENT_BFS_LIB environment variable); raises if it has not been built."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = os.environ.get("ENT_BFS_LIB") or LIB_PATH
+        if not os.path.exists(path):
             raise ImportError(
-                f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
-        handle = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+                f"{path} is missing: build it with __graft_entry__.build()")
+        handle = C.CDLL(path, mode=C.RTLD_GLOBAL)
         for name, res, args in SIGNATURES:
             fn = getattr(handle, name, None)
             if fn is None:

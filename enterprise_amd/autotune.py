@@ -37,6 +37,8 @@ SPACE = {
     "alpha": [4, 8, 15, 30, 60],
     "beta": [6, 12, 18, 36, 72],
     "queue_small": [-1, 1024, 4096, 16384, 65536],
+    "hub_cache": [0, 8, 10, 12],
+    "hub_td": [0, 1],
 }
 
 

@@ -126,6 +126,7 @@ struct Args {
   int chunk_words;
   int qcap_s;   // entries per queue stripe
   int q_small;  // largest frontier expanded from the queue
+  int q_maxdeg; // queue-driven levels need every frontier degree below this
   // LDS hub cache (hub_mask = entries - 1, or -1 when off): hub_slot[s] is
   // the highest-degree vertex v with v & hub_mask == s (or -1), hub_tag[s]
   // the level at which it joined the frontier (-1: not yet this search).
@@ -246,8 +247,9 @@ __device__ __forceinline__ long long xcd_wave_id() {
 
 // Records that v (degree d) joined the frontier of `level` if it is its
 // slot's hub.
+template <bool HUB>
 __device__ __forceinline__ void hub_note(const Args &a, int v, long long d, int level) {
-  if (a.hub_mask >= 0 && d >= a.hub_min) {
+  if (HUB && d >= a.hub_min) {
     const int s = v & a.hub_mask;
     if (a.hub_slot[s] == v) a.hub_tag[s] = level;
   }
@@ -372,7 +374,7 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
 // is vertex 64w + b). New frontier = marked and unvisited. Writes visited,
 // front[(L + 1) & 1] (every word), depth L + 1, clears the marks and counts
 // the new frontier into cnt[slot].
-template <int BS>
+template <int BS, bool HUB>
 __device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
   unsigned long long *fn = a.front[(L + 1) & 1];
   const int lane = lane_id();
@@ -393,7 +395,7 @@ __device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         nf += 1;
         mf += (unsigned long long)d;
         stage_push(sm, int(v), d);
-        hub_note(a, int(v), d, L + 1);
+        hub_note<HUB>(a, int(v), d, L + 1);
       }
       if (lane == 0 && nw) a.visited[w] = vis | nw;
     }
@@ -479,7 +481,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         nf += 1;
         mf += (unsigned long long)(e - s);
         stage_push(sm, v, e - s);
-        hub_note(a, v, e - s, L + 1);
+        hub_note<HUB>(a, v, e - s, L + 1);
       }
       unsigned long long lmask = __ballot(k < total && !found && p < e);
       while (lmask) {
@@ -513,7 +515,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
           nf += 1;
           mf += (unsigned long long)(ee - ss);
           stage_push(sm, vv, ee - ss);
-          hub_note(a, vv, ee - ss, L + 1);
+          hub_note<HUB>(a, vv, ee - ss, L + 1);
         }
       }
     }
@@ -530,7 +532,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
 }
 
 // Top-down level L from the striped queue (small frontiers whose vertices all
-// have degree < 4 * t_wave): one launch per level. A lane (or, for degree >=
+// have degree < queue_max_degree): one launch per level. A lane (or, for degree >=
 // t_wave, the whole wavefront) expands a frontier vertex; each unvisited
 // neighbour is claimed with an atomic OR on its visited word, and the
 // claiming lane writes its depth and stages it for the next queue.
@@ -608,7 +610,7 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
 
 // Rebuilds front[L & 1] from the depth array (depth == L), one wavefront per
 // bitmap word, when a queue-driven level is followed by a bitmap-driven one.
-template <int BS>
+template <int BS, bool HUB>
 __device__ void rebuild_phase(const Args &a, Smem<BS> &sm, int L) {
   unsigned long long *fr = a.front[L & 1];
   const int lane = lane_id();
@@ -618,7 +620,7 @@ __device__ void rebuild_phase(const Args &a, Smem<BS> &sm, int L) {
     const bool in = v < a.n && a.depth[v] == L;
     const unsigned long long word = __ballot(in);
     if (lane == 0) fr[w] = word;
-    if (in && a.hub_mask >= 0) hub_note(a, int(v), a.off[v + 1] - a.off[v], L);
+    if (HUB && in) hub_note<HUB>(a, int(v), a.off[v + 1] - a.off[v], L);
   }
 }
 
@@ -690,8 +692,7 @@ __device__ __forceinline__ void next_state(const Args &a, State &s, const unsign
   if (next_bu) {
     s.phase = bm ? PH_BU : PH_REBUILD;
     s.pending = PH_BU;
-  } else if (q && nf <= (unsigned long long)a.q_small &&
-             tot[K_MAXDEG] < (unsigned long long)min(a.t_block, 4 * a.t_wave)) {
+  } else if (q && nf <= (unsigned long long)a.q_small && tot[K_MAXDEG] < (unsigned long long)a.q_maxdeg) {
     s.phase = PH_TDQ;
   } else {
     s.phase = bm ? PH_TD : PH_REBUILD;
@@ -805,33 +806,31 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu) {
   }
 }
 
-template <int BS>
+template <int BS, bool HUB>
 __device__ __forceinline__ void run_phase(const Args &a, Smem<BS> &sm, long long i) {
   const int L = sm.S.level;
   const int slot = int(i % 3);
   switch (sm.S.phase) {
     case PH_TD:
-      if (a.hub_td && a.hub_mask >= 0) td_phase<BS, true>(a, sm, L);
+      if (HUB && a.hub_td) td_phase<BS, HUB>(a, sm, L);
       else td_phase<BS, false>(a, sm, L);
       break;
-    case PH_BU:
-      if (a.hub_mask >= 0) bu_phase<BS, true>(a, sm, L, slot);
-      else bu_phase<BS, false>(a, sm, L, slot);
-      break;
-    case PH_INSPECT: inspect_phase<BS>(a, sm, L, slot); break;
+    case PH_BU: bu_phase<BS, HUB>(a, sm, L, slot); break;
+    case PH_INSPECT: inspect_phase<BS, HUB>(a, sm, L, slot); break;
     case PH_TDQ: tdq_phase<BS>(a, sm, L, slot); break;
-    default: rebuild_phase<BS>(a, sm, L); break;
+    default: rebuild_phase<BS, HUB>(a, sm, L); break;
   }
 }
 
 // Launch i of a search: one phase. No end-of-launch synchronisation: the next
 // launch derives the next state itself.
-template <int BS>
+// HUB: the variant with the LDS hub table (only launched when it is on).
+template <int BS, bool HUB>
 __global__ __launch_bounds__(BS) void k_level(Args a, long long i) {
   __shared__ Smem<BS> sm;
   begin_launch(a, sm, i, false);
   if (sm.S.done) return;
-  run_phase<BS>(a, sm, i);
+  run_phase<BS, HUB>(a, sm, i);
 }
 
 // Two-level arrival: blocks count on their stripe, the last of each stripe on
@@ -884,13 +883,13 @@ __device__ __forceinline__ void grid_barrier(const Args &a) {
 
 // Whole search in one cooperative launch: launches 0, 1, 2, ... separated by
 // grid barriers. Stops on completion or on a barrier timeout.
-template <int BS>
+template <int BS, bool HUB>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_persist(Args a) {
   __shared__ Smem<BS> sm;
   for (long long i = 0;; ++i) {
     begin_launch(a, sm, i, true);
     if (sm.S.done || sm.S.error) break;
-    run_phase<BS>(a, sm, i);
+    run_phase<BS, HUB>(a, sm, i);
     grid_barrier(a);
     if (ld_agent(&a.ctl->st[0].error) & ERR_TIMEOUT) break;
   }
@@ -1055,6 +1054,7 @@ int check_config(const ent_bfs_config *c) {
   if (c->hub_cache != 0 && (c->hub_cache < 6 || c->hub_cache > 13))
     return set_error(ENT_EINVAL, "hub_cache %d not 0 or in [6, 13]", c->hub_cache);
   if (c->hub_td != 0 && c->hub_td != 1) return set_error(ENT_EINVAL, "hub_td %d not 0 or 1", c->hub_td);
+  if (c->queue_max_degree < 0) return set_error(ENT_EINVAL, "queue_max_degree %d", c->queue_max_degree);
   return ENT_OK;
 }
 
@@ -1077,6 +1077,7 @@ Args make_args(ent_graph *g, int *depth) {
   if (g->cfg.queue_small < 0) a.q_small = 0;
   else if (g->cfg.queue_small > 0) a.q_small = int(std::min<int64_t>(g->cfg.queue_small, int64_t(kStripes) * g->qcap_s / 2));
   else a.q_small = int(std::min<int64_t>(std::max<int64_t>(4096, g->n / 256), int64_t(kStripes) * g->qcap_s / 2));
+  a.q_maxdeg = g->cfg.queue_max_degree > 0 ? g->cfg.queue_max_degree : g->cfg.t_block;
   a.ctl = g->ctl;
   a.m = g->m;
   a.n_active = g->n_active;
@@ -1109,18 +1110,21 @@ struct KernelSet {
 };
 
 template <int BS>
-KernelSet kernel_set() {
-  return {reinterpret_cast<const void *>(&k_init<BS>), reinterpret_cast<const void *>(&k_level<BS>),
-          reinterpret_cast<const void *>(&k_persist<BS>), sizeof(Smem<BS>)};
+KernelSet kernel_set(bool hub) {
+  return {reinterpret_cast<const void *>(&k_init<BS>),
+          hub ? reinterpret_cast<const void *>(&k_level<BS, true>) : reinterpret_cast<const void *>(&k_level<BS, false>),
+          hub ? reinterpret_cast<const void *>(&k_persist<BS, true>)
+              : reinterpret_cast<const void *>(&k_persist<BS, false>),
+          sizeof(Smem<BS>)};
 }
 
-KernelSet kernels_for(int bs) {
+KernelSet kernels_for(int bs, bool hub) {
   switch (bs) {
-    case 64: return kernel_set<64>();
-    case 128: return kernel_set<128>();
-    case 512: return kernel_set<512>();
-    case 1024: return kernel_set<1024>();
-    default: return kernel_set<256>();
+    case 64: return kernel_set<64>(hub);
+    case 128: return kernel_set<128>(hub);
+    case 512: return kernel_set<512>(hub);
+    case 1024: return kernel_set<1024>(hub);
+    default: return kernel_set<256>(hub);
   }
 }
 
@@ -1183,7 +1187,7 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
   const ent_bfs_config &cfg = g->cfg;
   int rc = build_lists(g);
   if (rc) return rc;
-  const KernelSet ks = kernels_for(cfg.block_size);
+  const KernelSet ks = kernels_for(cfg.block_size, g->hub_slots > 0);
   if (ks.smem + hub_lds_bytes(cfg) > 65536)
     return set_error(ENT_EINVAL, "hub_cache %d with block_size %d needs %zu bytes of LDS per block (64 KiB max)",
                      cfg.hub_cache, cfg.block_size, ks.smem + hub_lds_bytes(cfg));
@@ -1347,6 +1351,7 @@ void ent_bfs_default_config(ent_bfs_config *c) {
   c->batch_alpha = 4;
   c->hub_cache = 0;
   c->hub_td = 0;
+  c->queue_max_degree = 0;
 }
 
 int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int64_t m, int device,

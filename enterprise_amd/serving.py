@@ -11,7 +11,10 @@ is by depth/status-array memory.
 
 Queries queue per graph. Whenever a slot is free, its worker takes up to
 max_batch queued queries (FIFO) and runs them as one multi-source search
-(ent_bfs_batch_device). Queries that arrive while other batches run are
+(ent_bfs_batch_device), or, when at most plain_max were waiting, as
+single-source searches one after another (ent_bfs_device): a multi-source
+level sweeps every vertex, which costs more than a few direction-optimizing
+single-source searches when the batch is that small. Queries that arrive while other batches run are
 admitted in the next free slot. Batches on different slots run concurrently
 on the device. Each query's result is its depth row in the slot's device
 buffer. The row's device checksum (ent_device_row_hash) is returned, and
@@ -87,14 +90,15 @@ class _GraphLane:
         self.copy_rows = copy_rows
         self.queue = collections.deque()
         self.cv = threading.Condition()
-        self.stats = {"batches": 0, "queries": 0, "reached_edges": 0, "busy_s": 0.0, "batch_sizes": []}
+        self.stats = {"batches": 0, "plain_batches": 0, "queries": 0, "reached_edges": 0, "busy_s": 0.0,
+                      "batch_sizes": []}
 
 
 class BFSServer:
     """Continuous-batching server for BFS queries over resident graphs."""
 
     def __init__(self, mem_budget_bytes=None, max_batch=64, max_levels=32, max_queue=100000,
-                 max_slots=4):
+                 max_slots=4, plain_max=4):
         if not 1 <= max_batch <= 512:
             raise ValueError("max_batch must be in [1, 512]")
         self.mem_budget = mem_budget_bytes
@@ -102,6 +106,7 @@ class BFSServer:
         self.max_levels = max_levels
         self.max_queue = max_queue
         self.max_slots = max_slots
+        self.plain_max = plain_max
         self.lanes = {}
         self.threads = []
         self._stop = False
@@ -218,7 +223,13 @@ class BFSServer:
                 q.batch_size = len(batch)
             roots = np.array([q.root for q in batch], dtype=np.int32)
             try:
-                st = slot.dg.bfs_batch_device(roots, slot.out)
+                if len(batch) <= self.plain_max:
+                    st = {"reached_edges": 0}
+                    for i, r in enumerate(roots):
+                        one = slot.dg.bfs_device(int(r), slot.out + i * n * 4)
+                        st["reached_edges"] += one["reached_edges"]
+                else:
+                    st = slot.dg.bfs_batch_device(roots, slot.out)
                 hashes = slot.dg.row_hash(slot.out, len(batch), n)
                 rows = None
                 if lane.copy_rows:
@@ -234,6 +245,7 @@ class BFSServer:
                 with lane.cv:
                     s = lane.stats
                     s["batches"] += 1
+                    s["plain_batches"] += int(len(batch) <= self.plain_max)
                     s["queries"] += len(batch)
                     s["reached_edges"] += st["reached_edges"]
                     s["busy_s"] += t1 - t0

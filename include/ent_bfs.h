@@ -85,6 +85,9 @@ typedef struct ent_bfs_config {
   int32_t hub_td;          /* with hub_cache: top-down flags hub neighbours in
                               LDS and marks each flagged hub once per block
                               (expander.cuh:96-109, :172-185) */
+  int32_t queue_max_degree; /* a frontier goes through the queue only if all
+                               its vertices have degree below this
+                               (0 = t_block) */
 } ent_bfs_config;
 
 typedef struct ent_stats {

@@ -52,7 +52,32 @@ def test_alone_equals_mix(two_graphs):
             assert np.array_equal(q.depth, alone[key]), (q.graph, q.root, q.batch_size)
             assert q.row_hash == int(row_hash_host(alone[key]))
         sizes = srv.lanes["big"].stats["batch_sizes"]
-        assert max(sizes) > 1          # queries were batched
+        assert max(sizes) > srv.plain_max  # the multi-source path served some batches
+    finally:
+        srv.close()
+
+
+def test_small_batches_use_single_source_path(two_graphs):
+    """A lone query runs as a single-source search and equals the result
+    alone; a burst larger than plain_max runs as one multi-source batch."""
+    hg, dg = two_graphs["big"]
+    srv = BFSServer(max_batch=64, max_levels=40, max_slots=1, plain_max=2)
+    srv.add_graph("big", dg, copy_rows=True)
+    roots = hg.pick_roots(40, 8)
+    try:
+        qs = [srv.submit("big", int(r)) for r in roots[:2]]
+        srv.start()
+        for q in qs:
+            q.wait(60)
+        st = srv.lanes["big"].stats
+        assert st["plain_batches"] == 1 and st["batch_sizes"] == [2]
+        srv.stop()
+        qs += [srv.submit("big", int(r)) for r in roots[2:]]
+        srv.start()
+        for q in qs:
+            q.wait(60)
+            assert np.array_equal(q.depth, dg.bfs(q.root)[0]), q.root
+        assert st["batch_sizes"][1] == 38 and st["plain_batches"] == 1
     finally:
         srv.close()
 
@@ -92,7 +117,7 @@ def test_memory_budget_admission(two_graphs):
 def test_failed_batch_keeps_slot_usable():
     g, _ = graphs.path(200)
     with DeviceGraph(g) as dg:
-        srv = BFSServer(max_batch=4, max_levels=50, max_slots=1)
+        srv = BFSServer(max_batch=4, max_levels=50, max_slots=1, plain_max=0)
         srv.add_graph("p", dg, copy_rows=True)
         srv.start()
         try:
@@ -108,7 +133,7 @@ def test_failed_batch_keeps_slot_usable():
             srv.close()
     h = HostGraph.rmat(10, 8, seed=5)
     with DeviceGraph(h) as dh:
-        srv = BFSServer(max_batch=4, max_levels=3, max_slots=1)
+        srv = BFSServer(max_batch=4, max_levels=3, max_slots=1, plain_max=0)
         srv.add_graph("r", dh, copy_rows=True)
         srv.start()
         try:
@@ -116,7 +141,7 @@ def test_failed_batch_keeps_slot_usable():
             with pytest.raises(L.EntError):
                 srv.submit("r", r).wait(60)          # R-MAT scale 10 needs > 3 levels
             srv.close()
-            srv = BFSServer(max_batch=4, max_levels=32, max_slots=1)
+            srv = BFSServer(max_batch=4, max_levels=32, max_slots=1, plain_max=0)
             srv.add_graph("r", dh, copy_rows=True)
             srv.start()
             q = srv.submit("r", r).wait(60)
