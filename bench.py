@@ -96,7 +96,9 @@ def bench_plain(args):
     g.scale_tag = scale
     log(f"[bench] R-MAT scale {scale}: n={g.n} m={g.m} generated in {time.perf_counter() - t0:.1f}s")
     roots = g.pick_roots(64, 1002)
-    dg = DeviceGraph(g, device=0)
+    t0 = time.perf_counter()
+    dg = DeviceGraph(g, device=0, keep_order=args.keep_order)
+    upload_s = time.perf_counter() - t0
     tuned = load_tuned(dg, f"rmat{scale}")
     if args.set:
         dg.configure(**{kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.set})
@@ -154,7 +156,8 @@ def bench_plain(args):
         "config": {"workload": f"plain single-source DO-BFS, R-MAT scale {scale} ef 16 "
                                f"(a=.57 b=.19 c=.19, seed {scale}), 64 roots (seed 1002)",
                    "n": g.n, "m_directed": g.m, "roots": 64,
-                   "launch": dg.config.launch, "direction": dg.config.direction},
+                   "launch": dg.config.launch, "direction": dg.config.direction,
+                   "rows": "caller order" if args.keep_order else "by neighbour degree class"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic.figure(f"plain_rmat{scale}"),
@@ -170,6 +173,7 @@ def bench_plain(args):
             "device_ms_mean": round(float(np.mean(dev_ms)), 4),
             "host_ms_p50": round(float(np.percentile(host_ms, 50)), 4),
             "levels_mean": round(float(np.mean([st["levels"] for (_, _, st) in used])), 2),
+            "upload_s": round(upload_s, 3),
             "roots_used": len(used),
             "parity": parity,
         },
@@ -194,10 +198,14 @@ def main():
     ap.add_argument("--rates", default="100,200,500,1000,2000,5000")
     ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--grid-side", type=int, default=4096)
+    ap.add_argument("--grid-forms", default="", help="comma-separated launch forms for --mode grid "
+                                                       "(stream,graph,persistent,levels; default all)")
     ap.add_argument("--traffic", default=None, metavar="PATH",
                     help="measure HBM bytes per search of --mode (plain or batch) with two "
                          "rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by a copy kernel; "
                          "writes the summary to PATH")
+    ap.add_argument("--keep-order", action="store_true",
+                    help="upload the adjacency rows in the caller's order (default: by neighbour degree class)")
     ap.add_argument("--set", action="append", default=[],
                     help="config override key=value (ent_bfs_config field), repeatable")
     args = ap.parse_args()

@@ -25,6 +25,7 @@ ENT_ECANCELED = -7
 
 DIR_AUTO, DIR_REFERENCE, DIR_TOPDOWN, DIR_BOTTOMUP = 0, 1, 2, 3
 LAUNCH_PERSISTENT, LAUNCH_LEVELS, LAUNCH_GRAPH, LAUNCH_STREAM = 0, 1, 2, 3
+UPLOAD_KEEP_ORDER = 1
 
 i32p = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
@@ -89,7 +90,10 @@ SIGNATURES = [
     ("ent_bfs_default_config", None, [C.POINTER(BfsConfig)]),
     ("ent_graph_upload", C.c_int, [i64p, i32p, C.c_int64, C.c_int64, C.c_int,
                                    C.POINTER(C.c_void_p)]),
+    ("ent_graph_upload_ex", C.c_int, [i64p, i32p, C.c_int64, C.c_int64, C.c_int, C.c_int32,
+                                      C.POINTER(C.c_void_p)]),
     ("ent_graph_free", C.c_int, [C.c_void_p]),
+    ("ent_graph_download", C.c_int, [C.c_void_p, i64p, i32p]),
     ("ent_graph_share", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("ent_graph_info", C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p]),
     ("ent_set_config", C.c_int, [C.c_void_p, C.POINTER(BfsConfig)]),

@@ -54,7 +54,7 @@ def batch(args):
     g = HostGraph.rmat(scale, 16, seed=scale)
     roots = g.pick_roots_mixed(k, k // 2, 1004)
     log(f"[batch] R-MAT scale {scale}: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f}s)")
-    dg = DeviceGraph(g)
+    dg = DeviceGraph(g, keep_order=getattr(args, "keep_order", False))
     from . import autotune
     tuned = autotune.load(f"rmat{scale}") or autotune.load("rmat22")
     if tuned:
@@ -138,6 +138,9 @@ def grid(args):
     dg = DeviceGraph(g)
     forms = {"stream": (L.LAUNCH_STREAM, 16), "graph": (L.LAUNCH_GRAPH, 16),
              "persistent": (L.LAUNCH_PERSISTENT, 4), "levels": (L.LAUNCH_LEVELS, 2)}
+    only = [f for f in (getattr(args, "grid_forms", "") or "").split(",") if f]
+    if only:
+        forms = {k: v for k, v in forms.items() if k in only}
     res = {}
     for name, (launch, nroots) in forms.items():
         dg.configure(launch=launch, graph_levels=96)

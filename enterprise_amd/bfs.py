@@ -12,6 +12,7 @@ import ctypes as C
 
 import numpy as np
 
+from . import _lib as L
 from ._lib import BatchStats, BfsConfig, Stats, check, i32p, i64p, lib
 
 
@@ -24,7 +25,9 @@ def default_config():
 class DeviceGraph:
     """A CSR graph resident in HBM plus its BFS workspace."""
 
-    def __init__(self, host_graph=None, beg_pos=None, adj=None, device=0, share=None):
+    def __init__(self, host_graph=None, beg_pos=None, adj=None, device=0, share=None, keep_order=False):
+        """keep_order: keep the caller's adjacency order on the device (by
+        default rows are reordered by descending neighbour degree class)."""
         if share is not None:
             # Second handle on share's resident CSR, with its own workspace.
             h = C.c_void_p()
@@ -40,8 +43,9 @@ class DeviceGraph:
         self.m = int(a.size)
         self.device = device
         h = C.c_void_p()
-        check(lib().ent_graph_upload(b.ctypes.data_as(i64p), a.ctypes.data_as(i32p),
-                                     self.n, self.m, int(device), C.byref(h)))
+        check(lib().ent_graph_upload_ex(b.ctypes.data_as(i64p), a.ctypes.data_as(i32p),
+                                        self.n, self.m, int(device),
+                                        L.UPLOAD_KEEP_ORDER if keep_order else 0, C.byref(h)))
         self._h = h
 
     @property
@@ -87,6 +91,13 @@ class DeviceGraph:
         check(lib().ent_bfs_device(self.handle, int(root),
                                    C.c_void_p(depth_dev) if depth_dev else None, C.byref(st)))
         return st.as_dict()
+
+    def download(self):
+        """(beg_pos int64[n+1], adj int32[m]) as held on the device."""
+        b = np.empty(self.n + 1, dtype=np.int64)
+        a = np.empty(max(self.m, 1), dtype=np.int32)
+        check(lib().ent_graph_download(self.handle, b.ctypes.data_as(i64p), a.ctypes.data_as(i32p)))
+        return b, a[:self.m]
 
     def depth_device_ptr(self):
         return lib().ent_depth_device_ptr(self.handle)

@@ -578,8 +578,29 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
         }
       }
       wave_sync();
-      if (valid) {
-        const int nb = min(64, a.k - 64 * g);
+      const int nb = min(64, a.k - 64 * g);
+      if ((a.n & 3) == 0) {
+        // 16-byte streaming stores: lane l writes vertices 4(l%16) .. +3 of
+        // the segment for source 4j + l/16, j = 0..15 (rows are 16-byte
+        // aligned when n is a multiple of 4).
+        const int q = lane & 15, sb = lane >> 4;
+        const long long v0 = wd * 64 + 4 * q;
+        if (v0 < a.n) {
+          for (int j = 0; j < 16; ++j) {
+            const int b = 4 * j + sb;
+            if (b >= nb) break;
+            typedef int v4i __attribute__((ext_vector_type(4)));
+            const unsigned char d0 = d8[(4 * q + 0) * 68 + b], d1 = d8[(4 * q + 1) * 68 + b];
+            const unsigned char d2 = d8[(4 * q + 2) * 68 + b], d3 = d8[(4 * q + 3) * 68 + b];
+            v4i x;
+            x.x = d0 == 0xFF ? -1 : int(d0);
+            x.y = d1 == 0xFF ? -1 : int(d1);
+            x.z = d2 == 0xFF ? -1 : int(d2);
+            x.w = d3 == 0xFF ? -1 : int(d3);
+            __builtin_nontemporal_store(x, reinterpret_cast<v4i *>(a.out + (long long)(64 * g + b) * a.n + v0));
+          }
+        }
+      } else if (valid) {
         for (int b = 0; b < nb; ++b) {
           const unsigned char d = d8[lane * 68 + b];
           a.out[(long long)(64 * g + b) * a.n + v] = d == 0xFF ? -1 : int(d);

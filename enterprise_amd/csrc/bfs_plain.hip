@@ -1356,7 +1356,13 @@ void ent_bfs_default_config(ent_bfs_config *c) {
 
 int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int64_t m, int device,
                      ent_graph **out) {
+  return ent_graph_upload_ex(beg_pos, adj, n, m, device, 0, out);
+}
+
+int ent_graph_upload_ex(const int64_t *beg_pos, const int32_t *adj, int64_t n, int64_t m, int device,
+                        int32_t flags, ent_graph **out) {
   clear_error();
+  if (flags & ~int32_t(ENT_UPLOAD_KEEP_ORDER)) return set_error(ENT_EINVAL, "unknown upload flags 0x%x", flags);
   if (!out) return set_error(ENT_EINVAL, "out is NULL");
   *out = nullptr;
   if (!beg_pos || (m > 0 && !adj)) return set_error(ENT_EINVAL, "NULL CSR array");
@@ -1433,6 +1439,7 @@ int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int6
   if (m) UP_TRY(hipMemcpy(g->adj, adj, sizeof(int) * size_t(m), hipMemcpyHostToDevice));
   UP_TRY(hipMemcpy(g->nzmask, nz.data(), sizeof(unsigned long long) * size_t(g->nwords), hipMemcpyHostToDevice));
   rc = alloc_workspace(g);
+  if (!rc && !(flags & ENT_UPLOAD_KEEP_ORDER)) rc = order_rows_by_degree(g);
   if (rc) {
     release(g);
     return rc;
@@ -1440,6 +1447,19 @@ int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int6
 #undef UP_TRY
 #undef UP_ALLOC
   *out = g;
+  return ENT_OK;
+}
+
+int ent_graph_download(ent_graph *g, int64_t *beg_pos, int32_t *adj) {
+  clear_error();
+  if (!g) return set_error(ENT_EINVAL, "NULL graph");
+  std::lock_guard<std::mutex> lk(g->mu);
+  HIP_TRY(hipSetDevice(g->device));
+  if (beg_pos)
+    HIP_TRY(hipMemcpyAsync(beg_pos, g->off, sizeof(int64_t) * size_t(g->n + 1), hipMemcpyDeviceToHost, g->stream));
+  if (adj && g->m)
+    HIP_TRY(hipMemcpyAsync(adj, g->adj, sizeof(int32_t) * size_t(g->m), hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
   return ENT_OK;
 }
 

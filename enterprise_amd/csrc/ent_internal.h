@@ -13,6 +13,7 @@
 #include "ent_bfs.h"
 #include "ent_common.h"
 
+struct ent_graph;
 namespace ent {
 struct Args;
 struct Control;
@@ -24,11 +25,10 @@ inline void ignore(T) {}
 // the last handle.
 struct GraphStore {
   int device = 0;
-  std::shared_ptr<ent::GraphStore> store;  // resident CSR (shared)
-  long long *off = nullptr;                 // = store->off
-  int *adj = nullptr;                       // = store->adj
-  unsigned long long *nzmask = nullptr;     // = store->nzmask
-  std::vector<int32_t> h_deg;  // host copy of degrees (for the static lists)
+  long long *off = nullptr;              // int64[n + 1]
+  int *adj = nullptr;                    // int32[m]
+  unsigned long long *nzmask = nullptr;  // bit v: degree(v) > 0
+  std::vector<int32_t> h_deg;            // host copy of degrees (for the static lists)
   ~GraphStore() {
     if (hipSetDevice(device) != hipSuccess) return;
     ignore(hipFree(off));
@@ -36,6 +36,9 @@ struct GraphStore {
     ignore(hipFree(nzmask));
   }
 };
+// Reorders every adjacency row by descending floor(log2(neighbour degree)),
+// stably (adj_order.hip).
+int order_rows_by_degree(ent_graph *g);
 }  // namespace ent
 
 struct ent_graph {

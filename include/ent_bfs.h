@@ -115,7 +115,22 @@ void ent_bfs_default_config(ent_bfs_config *cfg);
  * n < 2^31; adjacency ids must lie in [0, n). */
 int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n,
                      int64_t m, int device, ent_graph **out);
+
+/* Upload flags. By default the device copy of every adjacency row is
+ * reordered by descending floor(log2(neighbour degree)) (stable within a
+ * class): depths and counts do not depend on row order, and bottom-up scans
+ * meet frontier hubs first. ENT_UPLOAD_KEEP_ORDER keeps the caller's order
+ * (the reference's, expander.cuh:486-523). */
+#define ENT_UPLOAD_KEEP_ORDER 1
+
+/* ent_graph_upload with flags (ent_graph_upload = flags 0). */
+int ent_graph_upload_ex(const int64_t *beg_pos, const int32_t *adj, int64_t n,
+                        int64_t m, int device, int32_t flags, ent_graph **out);
 int ent_graph_free(ent_graph *g);
+
+/* Copies the device CSR back to host arrays (beg_pos: n + 1 entries, adj: m
+ * entries; either may be NULL), in the order the device holds it. */
+int ent_graph_download(ent_graph *g, int64_t *beg_pos, int32_t *adj);
 
 /* A second handle on the same resident graph: it shares src's device CSR
  * (no copy; the CSR is freed with the last handle) and has its own stream

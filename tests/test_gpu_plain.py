@@ -198,3 +198,28 @@ def test_hub_cache_rejects_bad_sizes(oracle, rmat16):
         _check(oracle, g, dg, int(g.pick_roots(1, 3)[0]), "rmat16/hub13td1/bs1024")
     finally:
         dg.set_config(base)
+
+
+def test_row_order_upload(oracle):
+    """Default upload reorders each row by descending floor(log2(neighbour
+    degree)), stably; ENT_UPLOAD_KEEP_ORDER keeps the caller's rows. Both give
+    the oracle's depths and counts."""
+    g = HostGraph.rmat(14, 16, seed=61)
+    deg = np.diff(g.beg_pos)
+    cls = np.where(deg > 1, np.floor(np.log2(np.maximum(deg, 1))), 0).astype(np.int64)
+    with DeviceGraph(g) as dg, DeviceGraph(g, keep_order=True) as dk:
+        b, a = dg.download()
+        assert np.array_equal(b, g.beg_pos)
+        bk, ak = dk.download()
+        assert np.array_equal(ak, g.adj)
+        for v in range(g.n):
+            s, e = int(b[v]), int(b[v + 1])
+            row, orig = a[s:e], g.adj[s:e]
+            want = orig[np.argsort(-cls[orig], kind="stable")]
+            assert np.array_equal(row, want), v
+        for r in g.pick_roots(4, 17):
+            for d in DIRS.values():
+                dg.configure(direction=d)
+                dk.configure(direction=d)
+                _check(oracle, g, dg, int(r), f"ordered/{d}")
+                _check(oracle, g, dk, int(r), f"kept/{d}")
