@@ -76,6 +76,35 @@ def cpu_baseline_serial(g, roots, budget_s, gpu_depths=None):
                       f"{os.cpu_count()} host cores visible"}, checked, mismatches
 
 
+def cpu_baseline_openmp(g, roots, budget_s, threads):
+    """Direction-optimizing OpenMP BFS of the oracle on `threads` host cores
+    over a bounded sample of the same roots."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ct
+    oracle = oracle_ct.Oracle()
+    rates = []
+    t_start = time.perf_counter()
+    for r in roots:
+        t0 = time.perf_counter()
+        d = oracle.bfs_do(g.beg_pos, g.adj, int(r), threads=threads)
+        dt = time.perf_counter() - t0
+        edges, _ = oracle.report(g.beg_pos, d)
+        if d.max() > 2:
+            rates.append(edges / dt / 1e9)
+        if time.perf_counter() - t_start > budget_s:
+            break
+    return {"value": round(hmean(rates), 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"OpenMP direction-optimizing BFS (oracle) on {threads} threads, {len(rates)} of the "
+                      f"{len(roots)} roots, R-MAT scale {g.scale_tag}, host {cpu_model()}"}
+
+
+def host_threads():
+    try:
+        return max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "0")) or 1 << 30))
+    except (AttributeError, ValueError):
+        return os.cpu_count() or 1
+
+
 def load_tuned(dg, name):
     path = os.path.join(ROOT, "enterprise_amd", "tuned", f"{name}.json")
     if os.path.exists(path):
@@ -122,9 +151,14 @@ def bench_plain(args):
     t_all = time.perf_counter() - t_all0
     del dg_sync
     log(f"[bench] timed {args.steps} BFS in {t_all:.3f}s")
-    # Depths of a few timed roots for the CPU check (outside the timed region).
+    # Depths of a few timed roots for the CPU check (outside the timed region);
+    # these calls copy the depth array to host memory, so their wall time is
+    # the PCIe-inclusive rate.
+    pcie = []
     for i in range(min(args.check_roots, 64)):
-        d, _ = dg.bfs(int(roots[i]))
+        t0 = time.perf_counter()
+        d, st = dg.bfs(int(roots[i]))
+        pcie.append(st["reached_edges"] / (time.perf_counter() - t0) / 1e9)
         gpu_depths[i] = d
     used = [(r, dt, st) for (r, dt, st) in per if st["levels"] - 1 > 2]
     gteps = [st["reached_edges"] / dt / 1e9 for (r, dt, st) in used]
@@ -136,10 +170,12 @@ def bench_plain(args):
     achieved_host = sum(alg_bytes) / (sum(host_ms) * 1e-3) / 1e9
     value = hmean(gteps)
     cpu = None
+    cpu_serial = None
     parity = None
     if not args.no_cpu:
-        cpu, checked, mism = cpu_baseline_serial(g, roots, args.cpu_budget, gpu_depths)
+        cpu_serial, checked, mism = cpu_baseline_serial(g, roots, args.cpu_budget / 2, gpu_depths)
         parity = {"roots_checked": checked, "mismatches": mism}
+        cpu = cpu_baseline_openmp(g, roots, args.cpu_budget / 2, host_threads())
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -176,6 +212,8 @@ def bench_plain(args):
             "upload_s": round(upload_s, 3),
             "roots_used": len(used),
             "parity": parity,
+            "cpu_serial_1core": cpu_serial,
+            "gteps_hmean_pcie_inclusive": round(hmean(pcie), 3) if pcie else None,
         },
     }
     dg.free()
