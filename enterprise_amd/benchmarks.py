@@ -142,8 +142,9 @@ def grid(args):
     if only:
         forms = {k: v for k, v in forms.items() if k in only}
     res = {}
+    extra = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in getattr(args, "set", []) or []}
     for name, (launch, nroots) in forms.items():
-        dg.configure(launch=launch, graph_levels=96)
+        dg.configure(launch=launch, graph_levels=96, **extra)
         dg.bfs_device(int(roots[0]))   # warm-up (graph capture, launch sizing)
         times, levels, phases = [], [], []
         for r in roots[:nroots]:

@@ -866,8 +866,10 @@ __device__ __forceinline__ void grid_barrier(const Args &a) {
     if (arrive_last(c)) {
       __hip_atomic_store(&c->gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
+      // Relaxed polling: an acquire load per poll would invalidate the
+      // caches on every iteration; the acquire fence below follows the flip.
       const uint64_t t0 = rt_clock();
-      while (__hip_atomic_load(&c->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+      while (ld_agent(&c->gen) == g) {
         __builtin_amdgcn_s_sleep(1);
         if (rt_clock() - t0 > 400000000ull) {
           atomicOr(&c->st[0].error, ERR_TIMEOUT);
