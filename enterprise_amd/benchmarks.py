@@ -41,6 +41,7 @@ def batch(args):
 
     import numpy as np
 
+    from . import _lib as L
     from . import traffic
     from .bfs import DeviceGraph
     from .graph import HostGraph
@@ -73,6 +74,9 @@ def batch(args):
             plain_rates.append(st["reached_edges"] / dt / 1e9)
         plain_hash.append(int(dg.row_hash(dg.depth_device_ptr(), 1, g.n)[0]))
     log(f"[batch] plain: {len(roots)} roots in {plain_time * 1e3:.1f} ms")
+    if getattr(args, "launch", None):  # launch form of the multi-source search only
+        dg.configure(launch={"persistent": L.LAUNCH_PERSISTENT, "levels": L.LAUNCH_LEVELS,
+                             "graph": L.LAUNCH_GRAPH, "stream": L.LAUNCH_STREAM}[args.launch])
     out = dg.device_alloc(k * g.n * 4)
     try:
         dg.batch_reserve(k, 48)
@@ -94,7 +98,8 @@ def batch(args):
         "mode": "batch", "metric": "aggregate GTEPS, 512-source batched vs plain",
         "value": round(batch_gteps, 2), "unit": "GTEPS", "dtype": "int32", "data": "synthetic",
         "config": {"workload": f"multi-source BFS, R-MAT scale {scale} ef 16 (seed {scale}), 512 roots "
-                               "(256 top-1% degree, 256 degree 1..4, seed 1004)", "k": k, "words": best["words"]},
+                               "(256 top-1% degree, 256 degree 1..4, seed 1004)", "k": k, "words": best["words"],
+                   "launch": getattr(args, "launch", None) or "stream"},
         "batch_ms": round(best["host_ms"], 3), "batch_device_ms": round(best["device_ms"], 3),
         "plain_aggregate_gteps": round(plain_agg, 2), "plain_hmean_gteps": round(_hmean(plain_rates), 2),
         "plain_total_ms": round(plain_time * 1e3, 2),

@@ -35,7 +35,7 @@
 namespace ent {
 
 enum BPhase : int { BPH_TD = 0, BPH_BU = 1, BPH_APPLY = 2, BPH_OUTPUT = 3 };
-enum : int { BERR_LEVELS = 8 };
+enum : int { BERR_LEVELS = 8, BERR_TIMEOUT = 16 };
 
 constexpr int kBStripes = 64;
 constexpr int kBStripeU64 = 16;
@@ -63,6 +63,7 @@ struct BState {
 
 struct BControl {
   BState st[2];
+  GridSync sync;  // persistent form only
   BCounters cnt[3];
 };
 
@@ -213,14 +214,15 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
 }
 
 template <int BS, int W>
-__device__ __forceinline__ void begin_blaunch(const BArgs &a, BSmem<BS> &sm, long long i) {
+__device__ __forceinline__ void begin_blaunch(const BArgs &a, BSmem<BS> &sm, long long i, bool coherent = false) {
   if (threadIdx.x < kWave) {
     BControl *c = a.ctl;
     const int lane = threadIdx.x;
     const BCounters &pc = c->cnt[(i + 2) % 3];
     unsigned long long t[14];
 #pragma unroll
-    for (int j = 0; j < 14; ++j) t[j] = pc.v[lane * kBStripeU64 + j];
+    for (int j = 0; j < 14; ++j)
+      t[j] = coherent ? ld_agent(&pc.v[lane * kBStripeU64 + j]) : pc.v[lane * kBStripeU64 + j];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
@@ -230,7 +232,15 @@ __device__ __forceinline__ void begin_blaunch(const BArgs &a, BSmem<BS> &sm, lon
       }
     }
     if (lane == 0) {
-      const BState S = next_bstate(a, c->st[(i + 1) & 1], t, W);
+      BState prev;
+      if (coherent) {
+        const unsigned *s32 = reinterpret_cast<const unsigned *>(&c->st[(i + 1) & 1]);
+        unsigned *d32 = reinterpret_cast<unsigned *>(&prev);
+        for (int w = 0; w < int(sizeof(BState) / 4); ++w) d32[w] = ld_agent(s32 + w);
+      } else {
+        prev = c->st[(i + 1) & 1];
+      }
+      const BState S = next_bstate(a, prev, t, W);
       sm.S = S;
       if (blockIdx.x == 0) c->st[i & 1] = S;
     }
@@ -646,6 +656,40 @@ __global__ __launch_bounds__(BS) void k_blevel_bu(BArgs a, long long i) {
 
 // Zeroes seen, lv[0], lv[1] and the joint frontier; clears counters; sets the
 // pseudo-level -1 state.
+// The whole multi-source search in one cooperative launch: launches 0, 1,
+// 2, ... of the per-phase form separated by grid barriers. Stops on
+// completion, or when a barrier wait exceeds ~4 s in any block (the search is
+// then marked failed). Exit decisions are taken from LDS after a block
+// barrier, so a block leaves the loop as a whole.
+template <int BS, int W>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W >= 8 ? 3 : 4))) void k_bpersist(BArgs a) {
+  __shared__ BSmem<BS> sm;
+  __shared__ int stop;
+  for (long long i = 0;; ++i) {
+    begin_blaunch<BS, W>(a, sm, i, true);
+    const BState &S = sm.S;
+    if (S.done || S.error) break;
+    const int slot = int(i % 3);
+    switch (S.phase) {
+      case BPH_TD: btd_phase<BS, W>(a, sm, S.level); break;
+      case BPH_APPLY: bapply_phase<BS, W>(a, sm, S.level); break;
+      case BPH_BU: bbu_phase<BS, W>(a, sm, S.level); break;
+      default: boutput_phase<BS, W>(a, sm, S.levels); break;
+    }
+    if (S.phase != BPH_TD) block_counts_flush(a, sm, slot);
+    const bool ok = grid_sync(&a.ctl->sync, 400000000ull);
+    if (threadIdx.x == 0) {
+      if (!ok) {
+        atomicOr(&a.ctl->st[0].error, BERR_TIMEOUT);
+        atomicOr(&a.ctl->st[1].error, BERR_TIMEOUT);
+      }
+      stop = !ok || (ld_agent(&a.ctl->st[0].error) & BERR_TIMEOUT) != 0;
+    }
+    __syncthreads();
+    if (stop) break;
+  }
+}
+
 __global__ void k_binit_zero(BArgs a) {
   const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long nt = (long long)gridDim.x * blockDim.x;
@@ -661,6 +705,11 @@ __global__ void k_binit_zero(BArgs a) {
   }
   constexpr long long kC = (long long)kBStripes * kBStripeU64;
   for (long long i = gt; i < 3 * kC; i += nt) a.ctl->cnt[i / kC].v[i % kC] = 0;
+  for (long long i = gt; i < kSyncStripes * 32; i += nt) a.ctl->sync.arrive[i] = 0;
+  if (gt == 0) {
+    a.ctl->sync.arrive_all = 0;
+    a.ctl->sync.gen = 0;
+  }
   if (gt == 0) {
     BState S{};
     S.level = -1;
@@ -781,13 +830,14 @@ int reserve(ent_graph *g, int k, int lcap) {
 }
 
 struct BKernels {
-  const void *gen, *bu;
+  const void *gen, *bu, *persist;
 };
 
 template <int W>
 BKernels blevel_kernels() {
   return {reinterpret_cast<const void *>(&k_blevel<kBatchBS, W>),
-          reinterpret_cast<const void *>(&k_blevel_bu<kBatchBS, W>)};
+          reinterpret_cast<const void *>(&k_blevel_bu<kBatchBS, W>),
+          reinterpret_cast<const void *>(&k_bpersist<kBatchBS, W>)};
 }
 
 BKernels blevel_for(int W) {
@@ -858,6 +908,27 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
                             g->stream));
   }
   const BKernels kl = blevel_for(W);
+  if (c.launch == ENT_LAUNCH_PERSISTENT) {
+    // Every block must be resident: at most 16 wavefronts and 8 workgroups
+    // per CU (the kernel is compiled for at least 3 or 4 wavefronts per
+    // SIMD), and within the occupancy the runtime reports.
+    int occ = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kl.persist, kBatchBS, 0));
+    int per_cu = std::min(std::min(occ, 16 / (kBatchBS / 64)), 8);
+    if (c.blocks_per_cu > 0) per_cu = std::min(per_cu, c.blocks_per_cu);
+    if (per_cu < 1) return set_error(ENT_EDEVICE, "persistent multi-source kernel cannot be resident");
+    int pgrid = g->cus * per_cu;
+    if (c.grid_blocks > 0) pgrid = std::min(pgrid, c.grid_blocks);
+    void *args[] = {&a};
+    HIP_TRY(hipLaunchCooperativeKernel(kl.persist, dim3(pgrid), dim3(kBatchBS), args, 0, g->stream));
+    HIP_TRY(hipMemcpyAsync(b->h_ctl, b->ctl, sizeof(BState) * 2, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    const BState &s0 = b->h_ctl->st[0], &s1 = b->h_ctl->st[1];
+    *fin = s0.phases >= s1.phases ? s0 : s1;
+    fin->error |= (s0.error | s1.error) & BERR_TIMEOUT;
+    g->last_grid = pgrid;
+    return ENT_OK;
+  }
   const int lgrid = c.grid_blocks > 0 ? c.grid_blocks : g->cus * std::max(1, c.blocks_per_cu > 0 ? c.blocks_per_cu : 8);
   long long i = 0;
   int kk = std::max(2, b->expect_launches);
@@ -951,6 +1022,8 @@ int ent_bfs_batch_device(ent_graph *g, const int32_t *roots, int32_t k, int32_t 
   if (fin.error & BERR_LEVELS)
     return set_error(ENT_ENOMEM, "batch needs more than %d joint levels; reserve more with ent_batch_reserve",
                      g->batch->lcap);
+  if (fin.error & BERR_TIMEOUT)
+    return set_error(ENT_EDEVICE, "persistent multi-source search: grid barrier wait timed out");
   if (fin.error) return set_error(ENT_EINTERNAL, "device reported error flags 0x%x", fin.error);
   if (st) {
     std::memset(st, 0, sizeof *st);

@@ -94,11 +94,8 @@ struct State {
 
 struct Control {
   State st[2];
-  unsigned int arrive_all;
-  unsigned int gen;
-  unsigned int pad[30];
-  unsigned int arrive[kStripes * 32];  // one 128 B line per stripe
-  Counters cnt[3];                     // counts produced by launch i: cnt[i % 3]
+  GridSync sync;    // persistent form only
+  Counters cnt[3];  // counts produced by launch i: cnt[i % 3]
 };
 
 struct Args {
@@ -794,15 +791,15 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu) {
   }
   if (gt == 0) a.qbuf[0][0] = root;
   for (long long s = gt; s <= a.hub_mask; s += nt) a.hub_tag[s] = a.hub_slot[s] == root ? 0 : -1;
-  for (long long i = gt; i < kStripes * 32; i += nt) c->arrive[i] = 0;
+  for (long long i = gt; i < kSyncStripes * 32; i += nt) c->sync.arrive[i] = 0;
   if (gt == 0) {
     State S{};
     S.level = -1;
     S.phase = first_bu ? PH_BU : PH_INSPECT;
     c->st[1] = S;
     c->st[0] = S;
-    c->arrive_all = 0;
-    c->gen = 0;
+    c->sync.arrive_all = 0;
+    c->sync.gen = 0;
   }
 }
 
@@ -833,67 +830,28 @@ __global__ __launch_bounds__(BS) void k_level(Args a, long long i) {
   run_phase<BS, HUB>(a, sm, i);
 }
 
-// Two-level arrival: blocks count on their stripe, the last of each stripe on
-// arrive_all; returns true for the single last block. Thread 0 only.
-__device__ __forceinline__ bool arrive_last(Control *c) {
-  const unsigned G = gridDim.x;
-  const unsigned s = blockIdx.x % kStripes;
-  const unsigned quota = (G - s + kStripes - 1) / kStripes;
-  const unsigned nstripes = G < kStripes ? G : kStripes;
-  unsigned *sc = &c->arrive[s * 32];
-  const unsigned t = __hip_atomic_fetch_add(sc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (t != quota - 1) return false;
-  st_agent(sc, 0u);
-  const unsigned t2 = __hip_atomic_fetch_add(&c->arrive_all, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (t2 != nstripes - 1) return false;
-  st_agent(&c->arrive_all, 0u);
-  return true;
-}
-
-// Grid barrier (two-level arrival, generation flag). Every wavefront first
-// waits for its own stores to complete (the block barrier alone does not), so
-// one agent-scope release per block (L2 write-back) publishes the whole
-// block's writes; after the generation flips, one agent-scope acquire per
-// block (L1/L2 invalidate) precedes the block's next reads. Bounded: a wait
-// longer than ~4 s marks the search failed and releases the waiter.
-__device__ __forceinline__ void grid_barrier(const Args &a) {
-  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are done
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    Control *c = a.ctl;
-    const unsigned g = ld_agent(&c->gen);
-    if (arrive_last(c)) {
-      __hip_atomic_store(&c->gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      // Relaxed polling: an acquire load per poll would invalidate the
-      // caches on every iteration; the acquire fence below follows the flip.
-      const uint64_t t0 = rt_clock();
-      while (ld_agent(&c->gen) == g) {
-        __builtin_amdgcn_s_sleep(1);
-        if (rt_clock() - t0 > 400000000ull) {
-          atomicOr(&c->st[0].error, ERR_TIMEOUT);
-          atomicOr(&c->st[1].error, ERR_TIMEOUT);
-          break;
-        }
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-}
-
 // Whole search in one cooperative launch: launches 0, 1, 2, ... separated by
-// grid barriers. Stops on completion or on a barrier timeout.
+// grid barriers. Stops on completion, or when a barrier wait exceeds ~4 s in
+// any block (the search is then marked failed). Every exit decision is taken
+// from LDS after a block barrier, so a block leaves the loop as a whole.
 template <int BS, bool HUB>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_persist(Args a) {
   __shared__ Smem<BS> sm;
+  __shared__ int stop;
   for (long long i = 0;; ++i) {
     begin_launch(a, sm, i, true);
     if (sm.S.done || sm.S.error) break;
     run_phase<BS, HUB>(a, sm, i);
-    grid_barrier(a);
-    if (ld_agent(&a.ctl->st[0].error) & ERR_TIMEOUT) break;
+    const bool ok = grid_sync(&a.ctl->sync, 400000000ull);
+    if (threadIdx.x == 0) {
+      if (!ok) {
+        atomicOr(&a.ctl->st[0].error, ERR_TIMEOUT);
+        atomicOr(&a.ctl->st[1].error, ERR_TIMEOUT);
+      }
+      stop = !ok || (ld_agent(&a.ctl->st[0].error) & ERR_TIMEOUT) != 0;
+    }
+    __syncthreads();
+    if (stop) break;
   }
 }
 

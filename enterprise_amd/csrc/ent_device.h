@@ -95,4 +95,66 @@ __device__ __forceinline__ void st_agent(T *p, T v) {
 // 100 MHz constant clock.
 __device__ __forceinline__ uint64_t rt_clock() { return wall_clock64(); }
 
+// Grid barrier state for cooperative (all blocks resident) launches: 64
+// striped arrival counters one 128-byte line apart, a global arrival counter
+// and a generation number. All zero before the first use; every use leaves
+// the counters at zero again.
+constexpr int kSyncStripes = 64;
+struct GridSync {
+  unsigned int arrive_all;
+  unsigned int gen;
+  unsigned int pad[30];
+  unsigned int arrive[kSyncStripes * 32];
+};
+
+// Two-level arrival: blocks count on their stripe, the last of each stripe on
+// arrive_all; returns true for the single last block. Thread 0 only.
+__device__ __forceinline__ bool grid_arrive_last(GridSync *c) {
+  const unsigned G = gridDim.x;
+  const unsigned s = blockIdx.x % kSyncStripes;
+  const unsigned quota = (G - s + kSyncStripes - 1) / kSyncStripes;
+  const unsigned nstripes = G < kSyncStripes ? G : kSyncStripes;
+  unsigned *sc = &c->arrive[s * 32];
+  const unsigned t = __hip_atomic_fetch_add(sc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t != quota - 1) return false;
+  st_agent(sc, 0u);
+  const unsigned t2 = __hip_atomic_fetch_add(&c->arrive_all, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t2 != nstripes - 1) return false;
+  st_agent(&c->arrive_all, 0u);
+  return true;
+}
+
+// Grid barrier. Every wavefront first waits for its own stores to complete
+// (the block barrier alone does not), so one agent-scope release per block
+// publishes the whole block's writes. Waiters poll the generation with
+// relaxed loads (an acquire load per poll would invalidate the caches on
+// every iteration) and take one agent-scope acquire after the flip. Bounded:
+// returns false (in every thread of the block) when the wait exceeded
+// `limit` ticks of rt_clock.
+__device__ __forceinline__ bool grid_sync(GridSync *c, uint64_t limit) {
+  __shared__ int sync_ok;
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are done
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sync_ok = 1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned g = ld_agent(&c->gen);
+    if (grid_arrive_last(c)) {
+      __hip_atomic_store(&c->gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      const uint64_t t0 = rt_clock();
+      while (ld_agent(&c->gen) == g) {
+        __builtin_amdgcn_s_sleep(1);
+        if (rt_clock() - t0 > limit) {
+          sync_ok = 0;
+          break;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  return sync_ok != 0;
+}
+
 }  // namespace ent

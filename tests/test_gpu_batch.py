@@ -114,3 +114,36 @@ def test_batch_device_hash(rmat16):
             assert h[i] == row_hash_host(d)
     finally:
         dg.device_free(ptr)
+
+
+@pytest.mark.parametrize("k", [1, 65, 130, 512])
+def test_batch_persistent_form(oracle, rmat16, k):
+    """The whole multi-source search in one cooperative launch gives the same
+    rows and counts as the per-phase launches."""
+    g, dg = rmat16
+    roots = g.pick_roots(k, 300 + k) if k <= 64 else np.r_[g.pick_roots(64, 7),
+                                                          g.pick_roots(k - 64, 8)]
+    base = dg.config
+    try:
+        dg.configure(launch=L.LAUNCH_STREAM)
+        want, st_s = dg.bfs_batch(roots)
+        dg.configure(launch=L.LAUNCH_PERSISTENT)
+        st = _check_batch(oracle, g, dg, roots, f"rmat16/persistent/k{k}")
+        got, _ = dg.bfs_batch(roots)
+        assert np.array_equal(got, want)
+        for key in ("levels", "reached_edges", "reached_vertices", "alg_bytes"):
+            assert st[key] == st_s[key], key
+    finally:
+        dg.set_config(base)
+
+
+@pytest.mark.parametrize("name", ["cycle257", "star5000", "grid37x23", "disjoint"])
+def test_batch_persistent_closed_forms(oracle, name):
+    g, depth_fn = graphs.CLOSED_FORMS[name]()
+    roots = np.unique(np.r_[0, g.n // 2, g.n - 1, np.arange(0, g.n, max(1, g.n // 40))])[:64]
+    with DeviceGraph(g) as dg:
+        dg.configure(launch=L.LAUNCH_PERSISTENT)
+        dg.batch_reserve(len(roots), 200)
+        out, _ = dg.bfs_batch(roots)
+        for i, r in enumerate(roots):
+            assert np.array_equal(out[i], np.asarray(depth_fn(int(r)), dtype=np.int32)), (name, r)
