@@ -74,6 +74,9 @@ def batch(args):
             plain_rates.append(st["reached_edges"] / dt / 1e9)
         plain_hash.append(int(dg.row_hash(dg.depth_device_ptr(), 1, g.n)[0]))
     log(f"[batch] plain: {len(roots)} roots in {plain_time * 1e3:.1f} ms")
+    tuned_batch = autotune.load(f"rmat{scale}_batch")
+    if tuned_batch:  # knobs tuned for the multi-source search (the plain runs above use the plain ones)
+        dg.configure(**{kk: v for kk, v in tuned_batch.items() if kk in autotune.BATCH_SPACE})
     if getattr(args, "launch", None):  # launch form of the multi-source search only
         dg.configure(launch={"persistent": L.LAUNCH_PERSISTENT, "levels": L.LAUNCH_LEVELS,
                              "graph": L.LAUNCH_GRAPH, "stream": L.LAUNCH_STREAM}[args.launch])
