@@ -63,7 +63,8 @@ class BatchStats(C.Structure):
                 ("phases", C.c_int32),
                 ("reached_edges", C.c_int64), ("reached_vertices", C.c_int64),
                 ("alg_bytes", C.c_int64), ("device_ms", C.c_double),
-                ("host_ms", C.c_double), ("bu_level_mask", C.c_uint64)]
+                ("host_ms", C.c_double), ("bu_level_mask", C.c_uint64),
+                ("td_level_mask", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

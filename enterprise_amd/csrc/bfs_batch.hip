@@ -38,9 +38,20 @@ enum BPhase : int { BPH_TD = 0, BPH_BU = 1, BPH_APPLY = 2, BPH_OUTPUT = 3 };
 enum : int { BERR_LEVELS = 8, BERR_TIMEOUT = 16 };
 
 constexpr int kBStripes = 64;
-constexpr int kBStripeU64 = 16;
-// Counter words within a stripe.
-enum : int { C_JWORDS = 0, C_MF = 1, C_ALG = 2, C_MUNF = 3, C_ACTIVE = 4, C_EDGES = 12, C_VERTS = 13 };
+constexpr int kBStripeU64 = 32;
+// Counter words within a stripe: C_MFW + w is the degree sum of the vertices
+// in the new frontier of the sources of word w.
+enum : int {
+  C_JWORDS = 0,
+  C_MF = 1,
+  C_ALG = 2,
+  C_MUNF = 3,
+  C_ACTIVE = 4,
+  C_EDGES = 12,
+  C_VERTS = 13,
+  C_MFW = 14,
+  C_COUNT = 22
+};
 
 struct BCounters {
   unsigned long long v[kBStripes * kBStripeU64];
@@ -53,8 +64,10 @@ struct BState {
   int error;
   int levels;
   int phases;
-  int pad[2];
-  unsigned long long bu_mask;
+  unsigned int pull_words;  // words of this level expanded bottom-up (bit w)
+  unsigned int push_words;  // words of this level expanded top-down
+  unsigned long long bu_mask;  // levels that pulled some words
+  unsigned long long td_mask;  // levels that pushed some words
   unsigned long long alg_bytes;
   unsigned long long total_edges;
   unsigned long long total_vertices;
@@ -114,7 +127,7 @@ struct BSmem {
   unsigned long long cand[NW][kWave];
   int excl[NW][kWave];
   unsigned char d8[NW][kWave * 68];  // output staging: 64 lanes x 64 sources (+4 pad)
-  unsigned long long c[14];
+  unsigned long long c[C_COUNT];
   BState S;
 };
 
@@ -126,14 +139,44 @@ __device__ __forceinline__ bool bitv(const unsigned long long *bm, int v) {
 
 template <int BS>
 __device__ __forceinline__ void block_counts_reset(BSmem<BS> &sm) {
-  if (threadIdx.x < 14) sm.c[threadIdx.x] = 0;
+  if (threadIdx.x < C_COUNT) sm.c[threadIdx.x] = 0;
 }
 
 // Reduces per-lane values over the wavefront and adds them to LDS.
+// Degree sums of the new frontier by source word, wave-uniform (scalar
+// registers): each step adds the wave sum of the lanes' degrees, each capped
+// at 2^25 so that 64 of them fit 32 bits (the sums only steer the direction
+// choice).
+template <int W>
+struct WordSums {
+  unsigned long long d[W];
+};
+
+template <int W>
+__device__ __forceinline__ WordSums<W> word_sums_zero() {
+  WordSums<W> z;
+#pragma unroll
+  for (int w = 0; w < W; ++w) z.d[w] = 0;
+  return z;
+}
+
+// Adds the degree of this lane's vertex to the words where x has bits. All
+// lanes of the wavefront must be active; invalid lanes pass x = 0.
+template <int W>
+__device__ __forceinline__ void word_sums_add(WordSums<W> &ws, const Mask<W> &x, long long deg) {
+  const unsigned int dc = (unsigned int)min(deg, (long long)(1 << 25));
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const unsigned int part = x.w[w] ? dc : 0u;
+    if (__ballot(part != 0)) ws.d[w] += (unsigned long long)(unsigned int)wave_sum(int(part));
+  }
+}
+
 template <int BS, int W>
 __device__ __forceinline__ void wave_counts(BSmem<BS> &sm, unsigned long long jw, unsigned long long mf,
                                             unsigned long long alg, unsigned long long munf, const Mask<W> &act,
-                                            unsigned long long edges, unsigned long long verts) {
+                                            unsigned long long edges, unsigned long long verts,
+                                            const WordSums<W> &mfw) {
   unsigned long long vals[6] = {jw, mf, alg, munf, edges, verts};
   Mask<W> ac = act;
 #pragma unroll
@@ -149,15 +192,17 @@ __device__ __forceinline__ void wave_counts(BSmem<BS> &sm, unsigned long long jw
     for (int j = 0; j < 6; ++j)
       if (vals[j]) atomicAdd(&sm.c[idx[j]], vals[j]);
 #pragma unroll
-    for (int w = 0; w < W; ++w)
+    for (int w = 0; w < W; ++w) {
       if (ac.w[w]) atomicOr(&sm.c[C_ACTIVE + w], ac.w[w]);
+      if (mfw.d[w]) atomicAdd(&sm.c[C_MFW + w], mfw.d[w]);
+    }
   }
 }
 
 template <int BS>
 __device__ __forceinline__ void block_counts_flush(const BArgs &a, BSmem<BS> &sm, int slot) {
   __syncthreads();
-  if (threadIdx.x < 14) {
+  if (threadIdx.x < C_COUNT) {
     const unsigned long long x = sm.c[threadIdx.x];
     if (x) {
       unsigned long long *dst = &a.ctl->cnt[slot].v[(blockIdx.x % kBStripes) * kBStripeU64 + threadIdx.x];
@@ -173,8 +218,8 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
   BState s = p;
   if (p.done) return s;
   s.phases = p.phases + 1;
-  if (p.phase == BPH_TD) {
-    s.phase = BPH_APPLY;
+  if (p.phase == BPH_TD) {  // pushed words settle in the pull phase, if any, else in APPLY
+    s.phase = p.pull_words ? BPH_BU : BPH_APPLY;
     return s;
   }
   if (p.phase == BPH_OUTPUT) {
@@ -185,7 +230,8 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
     return s;
   }
   const int L = p.level;  // -1 before level 0
-  if (L >= 0 && p.phase == BPH_BU && L < 64) s.bu_mask |= 1ull << L;
+  if (L >= 0 && p.pull_words && L < 64) s.bu_mask |= 1ull << L;
+  if (L >= 0 && p.push_words && L < 64) s.td_mask |= 1ull << L;
   s.alg_bytes += tot[C_ALG];
   if (tot[C_JWORDS] == 0) {
     s.levels = L + 1;
@@ -199,17 +245,28 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
   }
   for (int w = 0; w < 8; ++w) s.active[w] = w < W ? tot[C_ACTIVE + w] : 0ull;
   const unsigned long long mf = tot[C_MF], munf = tot[C_MUNF];
-  bool bu;
-  switch (a.mode) {
-    case ENT_DIR_TOPDOWN: bu = false; break;
-    case ENT_DIR_BOTTOMUP: bu = true; break;
-    case ENT_DIR_REFERENCE:
-      bu = p.phase == BPH_BU || (L + 1 >= 1 && 10ull * mf >= 3ull * (unsigned long long)a.m);
-      break;
-    default: bu = mf * (unsigned long long)a.alpha > munf;
+  // Direction per 64-source word: a word whose new frontier is large is
+  // pulled, a word that lags is pushed, so a pull can stop once the large
+  // words are covered instead of scanning every row for the lagging ones.
+  unsigned pull = 0, live = 0;
+  for (int w = 0; w < W; ++w) {
+    if (!s.active[w]) continue;
+    live |= 1u << w;
+    bool bu;
+    switch (a.mode) {
+      case ENT_DIR_TOPDOWN: bu = false; break;
+      case ENT_DIR_BOTTOMUP: bu = true; break;
+      case ENT_DIR_REFERENCE:  // the reference's one-way switch, for every word at once
+        bu = p.pull_words != 0 || (L + 1 >= 1 && 10ull * mf >= 3ull * (unsigned long long)a.m);
+        break;
+      default: bu = tot[C_MFW + w] * (unsigned long long)a.alpha > munf;
+    }
+    if (bu) pull |= 1u << w;
   }
   s.level = L + 1;
-  s.phase = bu ? BPH_BU : BPH_TD;
+  s.pull_words = pull;
+  s.push_words = live & ~pull;
+  s.phase = s.push_words ? BPH_TD : BPH_BU;
   return s;
 }
 
@@ -219,18 +276,21 @@ __device__ __forceinline__ void begin_blaunch(const BArgs &a, BSmem<BS> &sm, lon
     BControl *c = a.ctl;
     const int lane = threadIdx.x;
     const BCounters &pc = c->cnt[(i + 2) % 3];
-    unsigned long long t[14];
-#pragma unroll
-    for (int j = 0; j < 14; ++j)
-      t[j] = coherent ? ld_agent(&pc.v[lane * kBStripeU64 + j]) : pc.v[lane * kBStripeU64 + j];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-      for (int j = 0; j < 14; ++j) {
-        const unsigned long long x = __shfl_xor(t[j], o);
-        t[j] = (j >= C_ACTIVE && j < C_ACTIVE + 8) ? (t[j] | x) : (t[j] + x);
+    // Totals of the 64 stripes: lane j sums (or ORs) counter j over the
+    // stripes with independent loads, so no cross-lane steps are needed.
+    __shared__ unsigned long long t[C_COUNT];
+    if (lane < C_COUNT) {
+      const bool is_or = lane >= C_ACTIVE && lane < C_ACTIVE + 8;
+      unsigned long long acc = 0;
+#pragma unroll 16
+      for (int st = 0; st < kBStripes; ++st) {
+        const unsigned long long *q = &pc.v[st * kBStripeU64 + lane];
+        const unsigned long long x = coherent ? ld_agent(q) : *q;
+        acc = is_or ? (acc | x) : (acc + x);
       }
+      t[lane] = acc;
     }
+    wave_sync();
     if (lane == 0) {
       BState prev;
       if (coherent) {
@@ -244,7 +304,7 @@ __device__ __forceinline__ void begin_blaunch(const BArgs &a, BSmem<BS> &sm, lon
       sm.S = S;
       if (blockIdx.x == 0) c->st[i & 1] = S;
     }
-    if (blockIdx.x == 0 && lane < 14)
+    if (blockIdx.x == 0 && lane < C_COUNT)
       for (int s = 0; s < kBStripes; ++s) c->cnt[(i + 1) % 3].v[s * kBStripeU64 + lane] = 0;
   }
   block_counts_reset(sm);
@@ -252,6 +312,18 @@ __device__ __forceinline__ void begin_blaunch(const BArgs &a, BSmem<BS> &sm, lon
 }
 
 // -- phases ------------------------------------------------------------------
+
+// Keeps the words of f that this level pushes; true if any bit is left.
+template <int W>
+__device__ __forceinline__ bool push_mask(Mask<W> &f, unsigned words) {
+  bool any = false;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    if (!((words >> w) & 1u)) f.w[w] = 0;
+    any |= f.w[w] != 0;
+  }
+  return any;
+}
 
 template <int W>
 __device__ __forceinline__ void push_edge(const BArgs &a, unsigned long long *dst, int u, const Mask<W> &f) {
@@ -274,17 +346,20 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const unsigned long long *fs = level_ptr(a, L);
   unsigned long long *dst = level_ptr(a, L + 1);
   const unsigned long long *jf = a.jf[L & 1];
+  const unsigned pw = sm.S.push_words;  // only these words are pushed at this level
   for (int p = blockIdx.x; p < a.n_pieces; p += gridDim.x) {
     const int v = a.piece_v[p];
     if (!bitv(jf, v)) continue;
-    const Mask<W> f = load_mask<W>(fs, v);
+    Mask<W> f = load_mask<W>(fs, v);
+    if (!push_mask<W>(f, pw)) continue;
     const long long s = a.off[v] + (long long)a.piece_k[p] * a.piece;
     push_range<W>(a, dst, s, min(a.off[v + 1], s + a.piece), threadIdx.x, BS, f);
   }
   for (int h = blockIdx.x; h < a.n_hub_block; h += gridDim.x) {
     const int v = a.hub_block[h];
     if (!bitv(jf, v)) continue;
-    const Mask<W> f = load_mask<W>(fs, v);
+    Mask<W> f = load_mask<W>(fs, v);
+    if (!push_mask<W>(f, pw)) continue;
     push_range<W>(a, dst, a.off[v], a.off[v + 1], threadIdx.x, BS, f);
   }
   const int wid = threadIdx.x / kWave, lane = lane_id();
@@ -318,8 +393,8 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
         s = a.off[v];
         e = a.off[v + 1];
         if (e - s < a.t_wave) {
-          const Mask<W> f = load_mask<W>(fs, v);
-          push_range<W>(a, dst, s, e, 0, 1, f);
+          Mask<W> f = load_mask<W>(fs, v);
+          if (push_mask<W>(f, pw)) push_range<W>(a, dst, s, e, 0, 1, f);
         }
       }
       const long long d = e - s;
@@ -329,8 +404,8 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
         wm &= wm - 1;
         const int vv = __shfl(v, src);
         const long long ss = __shfl(s, src), ee = __shfl(e, src);
-        const Mask<W> f = load_mask<W>(fs, vv);
-        push_range<W>(a, dst, ss, ee, lane, kWave, f);
+        Mask<W> f = load_mask<W>(fs, vv);
+        if (push_mask<W>(f, pw)) push_range<W>(a, dst, ss, ee, lane, kWave, f);
       }
     }
     wave_sync();
@@ -343,8 +418,10 @@ template <int W>
 __device__ __forceinline__ void settle_vertex(const BArgs &a, int L, bool valid, long long v, const Mask<W> &sd,
                                               const Mask<W> &x, long long deg, const unsigned long long *active,
                                               unsigned long long &jw, unsigned long long &mf, unsigned long long &alg,
-                                              unsigned long long &munf, Mask<W> &act, bool &any) {
+                                              unsigned long long &munf, Mask<W> &act, WordSums<W> &mfw,
+                                              bool &any) {
   any = false;
+  word_sums_add<W>(mfw, x, deg);  // x = 0 on invalid lanes
   if (!valid) return;
   int nzw = 0;
   Mask<W> ns;
@@ -378,6 +455,7 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
+  WordSums<W> mfw = word_sums_zero<W>();
   Mask<W> act;
 #pragma unroll
   for (int w = 0; w < W; ++w) act.w[w] = 0;
@@ -394,29 +472,30 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
       deg = a.off[v + 1] - a.off[v];
     }
     bool any;
-    settle_vertex<W>(a, L, valid, v, sd, x, deg, sm.S.active, jw, mf, alg, munf, act, any);
+    settle_vertex<W>(a, L, valid, v, sd, x, deg, sm.S.active, jw, mf, alg, munf, act, mfw, any);
     const unsigned long long bits = __ballot(any);
     if (lane == 0) jn[wd] = bits;
   }
-  wave_counts<BS, W>(sm, jw, mf, alg, munf, act, 0, 0);
+  wave_counts<BS, W>(sm, jw, mf, alg, munf, act, 0, 0, mfw);
 }
 
-// Bottom-up tail for the vertex of this lane. x = new bits (a subset of tgt,
-// the active sources that had not seen v), so the vertex stays unfinished
-// for some active source iff tgt has bits outside x. The seen mask is read
-// back only for vertices that gained bits. act is wave-uniform: it is merged
-// with a wave OR only when some lane holds a source not yet in it.
+// Bottom-up tail for the vertex of this lane. x = new bits: pulled bits of
+// the pull words plus, when this level also pushed, the bits pushed to v.
+// Both lie in ft = active & ~seen(v), whose popcount is ftc, so v stays
+// unfinished for some active source iff x has fewer bits. The seen mask is
+// read back only for vertices that gained bits. act is wave-uniform: it is
+// merged with a wave OR only when some lane holds a source not yet in it.
 template <int W>
-__device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, long long v, const Mask<W> &tgt,
+__device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, long long v, int ftc,
                                          const Mask<W> &x, long long deg, unsigned long long &jw,
                                          unsigned long long &mf, unsigned long long &alg, unsigned long long &munf,
-                                         Mask<W> &act, bool &any) {
-  int nzw = 0;
-  bool unf = false, fresh = false;
+                                         Mask<W> &act, WordSums<W> &mfw, bool &any) {
+  int nzw = 0, xc = 0;
+  bool fresh = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     nzw += x.w[w] != 0;
-    unf |= (tgt.w[w] & ~x.w[w]) != 0;
+    xc += __popcll(x.w[w]);
     fresh |= (x.w[w] & ~act.w[w]) != 0;
   }
   any = nzw != 0;
@@ -424,6 +503,7 @@ __device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, lon
 #pragma unroll
     for (int w = 0; w < W; ++w) act.w[w] |= wave_or64(x.w[w]);
   }
+  word_sums_add<W>(mfw, x, deg);  // x = 0 on invalid lanes
   if (!valid) return;
   if (any) {
     Mask<W> ns = load_mask<W>(a.seen, v);
@@ -434,7 +514,7 @@ __device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, lon
     mf += (unsigned long long)deg;
     alg += (unsigned long long)nzw * (8ull + 12ull * (unsigned long long)deg);
   }
-  if (unf) munf += (unsigned long long)deg;
+  if (xc < ftc) munf += (unsigned long long)deg;
   Mask<W> z;
 #pragma unroll
   for (int w = 0; w < W; ++w) z.w[w] = 0;
@@ -443,9 +523,11 @@ __device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, lon
 
 // Bottom-up (pull) level: every vertex v gathers the level-L masks of its
 // neighbours (only those in the joint frontier) until the OR covers
-// tgt = active & ~seen(v) or the list ends. Lanes probe their own first
-// `probe` neighbours two at a time; longer remaining lists are scanned by the
-// whole wavefront, one vertex at a time, with a wave OR per 64 neighbours.
+// tgt = active & ~seen(v) restricted to the pull words, or the list ends.
+// Lanes probe their own first `probe` neighbours two at a time; longer
+// remaining lists are scanned by the whole wavefront, one vertex at a time,
+// with a wave OR per 64 neighbours. When this level also pushed some words
+// (a top-down launch ran just before), the bits pushed to v are merged in.
 template <int BS, int W>
 __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const unsigned long long *fs = level_ptr(a, L);
@@ -456,13 +538,17 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
   Mask<W> act;
+  WordSums<W> mfw = word_sums_zero<W>();
 #pragma unroll
   for (int w = 0; w < W; ++w) act.w[w] = 0;
+  const unsigned pull = sm.S.pull_words;
+  const bool merge = sm.S.push_words != 0;
   for (long long wd = (long long)blockIdx.x * (BS / kWave) + threadIdx.x / kWave; wd < a.nwords; wd += nwv) {
     const long long v = wd * 64 + lane;
     const bool valid = v < a.n;
     Mask<W> tgt, acc;
     bool need = false;
+    int ftc = 0;
     long long s = 0, e = 0, p = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) tgt.w[w] = acc.w[w] = 0;
@@ -472,7 +558,9 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
       e = a.off[v + 1];
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        tgt.w[w] = sm.S.active[w] & ~sd.w[w];
+        const unsigned long long ft = sm.S.active[w] & ~sd.w[w];
+        ftc += __popcll(ft);
+        tgt.w[w] = ((pull >> w) & 1u) ? ft : 0ull;
         need |= tgt.w[w] != 0;
       }
       need = need && e > s;
@@ -542,13 +630,18 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     Mask<W> x;
 #pragma unroll
     for (int w = 0; w < W; ++w) x.w[w] = acc.w[w] & tgt.w[w];
+    if (valid && merge) {  // bits pushed to v by this level's top-down launch
+      const Mask<W> xp = load_mask<W>(nx, v);
+#pragma unroll
+      for (int w = 0; w < W; ++w) x.w[w] |= xp.w[w];
+    }
     if (valid) store_mask<W>(nx, v, x);
     bool any;
-    settle_bu<W>(a, L, valid, v, tgt, x, e - s, jw, mf, alg, munf, act, any);
+    settle_bu<W>(a, L, valid, v, ftc, x, e - s, jw, mf, alg, munf, act, mfw, any);
     const unsigned long long bits = __ballot(any);
     if (lane == 0) jn[wd] = bits;
   }
-  wave_counts<BS, W>(sm, jw, mf, alg, munf, act, 0, 0);
+  wave_counts<BS, W>(sm, jw, mf, alg, munf, act, 0, 0, mfw);
 }
 
 // Output: per 64-vertex segment, replay the level masks of each 64-source
@@ -622,7 +715,7 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
   Mask<W> none;
 #pragma unroll
   for (int w = 0; w < W; ++w) none.w[w] = 0;
-  wave_counts<BS, W>(sm, 0, 0, 0, 0, none, edges, verts);
+  wave_counts<BS, W>(sm, 0, 0, 0, 0, none, edges, verts, word_sums_zero<W>());
 }
 
 // Launch i runs as two kernels queued back to back: the bottom-up phase has
@@ -719,9 +812,14 @@ __global__ void k_binit_zero(BArgs a) {
   }
 }
 
-// Sets the source bits (level 0) and the level-0 counts computed on the host.
-__global__ void k_binit_src(BArgs a, const int *roots, unsigned long long jw, unsigned long long mf,
-                            unsigned long long alg, unsigned long long munf) {
+// Level-0 counts computed on the host (frontier 0 = the roots).
+struct BInit {
+  unsigned long long jw, mf, alg, munf;
+  unsigned long long mfw[8];  // degree sum of the distinct roots of word w
+};
+
+// Sets the source bits (level 0) and the level-0 counts.
+__global__ void k_binit_src(BArgs a, const int *roots, BInit in) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < a.k) {
     const int v = roots[s];
@@ -734,10 +832,11 @@ __global__ void k_binit_src(BArgs a, const int *roots, unsigned long long jw, un
   }
   if (s == 0) {
     unsigned long long *c = a.ctl->cnt[2].v;
-    c[C_JWORDS] = jw;
-    c[C_MF] = mf;
-    c[C_ALG] = alg;
-    c[C_MUNF] = munf;
+    c[C_JWORDS] = in.jw;
+    c[C_MF] = in.mf;
+    c[C_ALG] = in.alg;
+    c[C_MUNF] = in.munf;
+    for (int w = 0; w < 8; ++w) c[C_MFW + w] = in.mfw[w];
   }
 }
 
@@ -891,10 +990,15 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   vw.erase(std::unique(vw.begin(), vw.end()), vw.end());
   std::sort(vs.begin(), vs.end());
   vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
-  unsigned long long mf = 0, alg = 0;
-  for (int32_t v : vs) mf += (unsigned long long)g->store->h_deg[size_t(v)];
-  for (auto &p : vw) alg += 8ull + 12ull * (unsigned long long)g->store->h_deg[size_t(p.first)];
-  unsigned long long munf = (unsigned long long)g->m;  // every vertex still needs sources
+  BInit init{};
+  for (int32_t v : vs) init.mf += (unsigned long long)g->store->h_deg[size_t(v)];
+  for (auto &p : vw) {
+    const unsigned long long d = (unsigned long long)g->store->h_deg[size_t(p.first)];
+    init.alg += 8ull + 12ull * d;
+    init.mfw[p.second] += d;
+  }
+  init.jw = vw.size();
+  init.munf = (unsigned long long)g->m;  // every vertex still needs sources
   HIP_TRY(hipMemcpyAsync(b->roots, roots, sizeof(int32_t) * size_t(k), hipMemcpyHostToDevice, g->stream));
   const int grid = g->cus * 8;
   {
@@ -902,8 +1006,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
     HIP_TRY(hipLaunchKernel(reinterpret_cast<const void *>(&k_binit_zero), dim3(grid), dim3(256), args, 0, g->stream));
   }
   {
-    unsigned long long jw = vw.size();
-    void *args[] = {&a, &b->roots, &jw, &mf, &alg, &munf};
+    void *args[] = {&a, &b->roots, &init};
     HIP_TRY(hipLaunchKernel(reinterpret_cast<const void *>(&k_binit_src), dim3((k + 255) / 256), dim3(256), args, 0,
                             g->stream));
   }
@@ -1038,6 +1141,7 @@ int ent_bfs_batch_device(ent_graph *g, const int32_t *roots, int32_t k, int32_t 
     if (hipEventElapsedTime(&ms, g->ev0, g->ev1) == hipSuccess) st->device_ms = ms;
     st->host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     st->bu_level_mask = fin.bu_mask;
+    st->td_level_mask = fin.td_mask;
   }
   return ENT_OK;
 }

@@ -39,7 +39,11 @@ typedef struct ent_batch_stats {
                                summed over groups */
   double device_ms;         /* HIP-event time of the batch */
   double host_ms;           /* wall time of the call */
-  uint64_t bu_level_mask;   /* bit l: joint level l ran bottom-up */
+  uint64_t bu_level_mask;   /* bit l: level l pulled (bottom-up) some of
+                               its 64-source words */
+  uint64_t td_level_mask;   /* bit l: level l pushed (top-down) some of its
+                               words; both bits set: the level split its words
+                               by frontier size */
 } ent_batch_stats;
 
 /* Reserves the workspace for up to k sources and max_levels joint levels

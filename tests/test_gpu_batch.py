@@ -147,3 +147,24 @@ def test_batch_persistent_closed_forms(oracle, name):
         out, _ = dg.bfs_batch(roots)
         for i, r in enumerate(roots):
             assert np.array_equal(out[i], np.asarray(depth_fn(int(r)), dtype=np.int32)), (name, r)
+
+
+def test_batch_split_levels(oracle, rmat16):
+    """Half the roots from the top-1% degree vertices, half of degree 1..4:
+    the hub words are pulled while the lagging words are pushed at the same
+    level (both level bits set), and every row still equals the single-source
+    result, in both launch forms and with forced directions."""
+    g, dg = rmat16
+    roots = g.pick_roots_mixed(512, 256, 2024)
+    base = dg.config
+    try:
+        for launch in (L.LAUNCH_STREAM, L.LAUNCH_PERSISTENT):
+            dg.configure(launch=launch, direction=L.DIR_AUTO)
+            st = _check_batch(oracle, g, dg, roots, f"rmat16/split/{launch}")
+            assert st["bu_level_mask"] & st["td_level_mask"], st
+        for d in (L.DIR_TOPDOWN, L.DIR_BOTTOMUP, L.DIR_REFERENCE):
+            dg.configure(launch=L.LAUNCH_STREAM, direction=d)
+            st = _check_batch(oracle, g, dg, roots, f"rmat16/split/dir{d}")
+            assert not (st["bu_level_mask"] & st["td_level_mask"]), st
+    finally:
+        dg.set_config(base)
