@@ -95,6 +95,8 @@ SIGNATURES = [
                                       C.POINTER(C.c_void_p)]),
     ("ent_graph_free", C.c_int, [C.c_void_p]),
     ("ent_graph_download", C.c_int, [C.c_void_p, i64p, i32p]),
+    ("ent_bfs_parents_device", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("ent_bfs_parents", C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, C.POINTER(Stats)]),
     ("ent_graph_share", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("ent_graph_info", C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p]),
     ("ent_set_config", C.c_int, [C.c_void_p, C.POINTER(BfsConfig)]),

@@ -85,6 +85,20 @@ class DeviceGraph:
         check(lib().ent_bfs(self.handle, int(root), ptr, C.byref(st)))
         return out, st.as_dict()
 
+    def bfs_parents(self, root):
+        """One BFS; returns (depth int32[n], parent int32[n], stats)."""
+        d = np.empty(self.n, dtype=np.int32)
+        p = np.empty(self.n, dtype=np.int32)
+        st = Stats()
+        check(lib().ent_bfs_parents(self.handle, int(root), d.ctypes.data_as(i32p), p.ctypes.data_as(i32p),
+                                    C.byref(st)))
+        return d, p, st.as_dict()
+
+    def parents_device(self, root, depth_dev, parent_dev):
+        """Parents from a device depth array into a device parent array."""
+        check(lib().ent_bfs_parents_device(self.handle, int(root), C.c_void_p(depth_dev),
+                                           C.c_void_p(parent_dev)))
+
     def bfs_device(self, root, depth_dev=None):
         """One BFS writing to device memory (default: the handle's buffer)."""
         st = Stats()

@@ -893,6 +893,8 @@ void release(ent_graph *g) {
   ignore(hipFree(g->piece_v));
   ignore(hipFree(g->piece_k));
   ignore(hipFree(g->depth));
+  ignore(hipFree(g->aux));
+  ignore(hipFree(g->parent));
   ignore(hipFree(g->visited));
   ignore(hipFree(g->front[0]));
   ignore(hipFree(g->front[1]));
@@ -1251,7 +1253,8 @@ int alloc_workspace(ent_graph *g) {
       (rc = dev_alloc(g, reinterpret_cast<void **>(&g->mark), size_t(n))) ||
       (rc = dev_alloc(g, reinterpret_cast<void **>(&g->qbuf[0]), sizeof(int) * size_t(kStripes) * size_t(g->qcap_s))) ||
       (rc = dev_alloc(g, reinterpret_cast<void **>(&g->qbuf[1]), sizeof(int) * size_t(kStripes) * size_t(g->qcap_s))) ||
-      (rc = dev_alloc(g, reinterpret_cast<void **>(&g->ctl), sizeof(Control))))
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&g->ctl), sizeof(Control))) ||
+      (rc = dev_alloc(g, &g->aux, 256)))
     return rc;
   HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_ctl), sizeof(Control), hipHostMallocDefault));
   HIP_TRY(hipMemset(g->ctl, 0, sizeof(Control)));

@@ -60,6 +60,8 @@ struct ent_graph {
   int hub_min = 0;          // smallest degree in the table
   int lists_hub = -1, lists_hub_tw = -1;
   int *depth = nullptr;
+  void *aux = nullptr;     // 256 bytes of device scratch for host-side calls (counters)
+  int *parent = nullptr;   // parent array of ent_bfs_parents (allocated on first use)
   unsigned long long *visited = nullptr;
   unsigned long long *front[2] = {nullptr, nullptr};
   unsigned char *mark = nullptr;

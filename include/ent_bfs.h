@@ -128,6 +128,22 @@ int ent_graph_upload_ex(const int64_t *beg_pos, const int32_t *adj, int64_t n,
                         int64_t m, int device, int32_t flags, ent_graph **out);
 int ent_graph_free(ent_graph *g);
 
+/* Parents of a search from `root` whose depth array is depth_dev (device
+ * pointers, n entries each): parent[root] = root; parent[v] = the first
+ * neighbour u of v, in device row order, with depth[u] = depth[v] - 1; -1 for
+ * unreached vertices. The result is a BFS tree that passes the Graph500
+ * parent checks. Returns ENT_EINVAL (parents of the offending vertices set to
+ * -2) when some reached vertex has no neighbour one level closer, i.e. the
+ * depths are not a BFS from root. No reference counterpart: the reference
+ * outputs depths only (bfs_gpu_opt.cuh:446). */
+int ent_bfs_parents_device(ent_graph *g, int32_t root, const int32_t *depth_dev,
+                           int32_t *parent_dev);
+
+/* One search from root, then its parents; depth_host and parent_host (n
+ * entries, either may be NULL) receive both arrays. */
+int ent_bfs_parents(ent_graph *g, int32_t root, int32_t *depth_host,
+                    int32_t *parent_host, ent_stats *st);
+
 /* Copies the device CSR back to host arrays (beg_pos: n + 1 entries, adj: m
  * entries; either may be NULL), in the order the device holds it. */
 int ent_graph_download(ent_graph *g, int64_t *beg_pos, int32_t *adj);
