@@ -903,6 +903,10 @@ void release(ent_graph *g) {
   ignore(hipFree(g->qbuf[1]));
   ignore(hipFree(g->ctl));
   if (g->h_ctl) ignore(hipHostFree(g->h_ctl));
+  for (int b = 0; b < 2; ++b) {
+    if (g->stage[b]) ignore(hipHostFree(g->stage[b]));
+    if (g->stage_ev[b]) ignore(hipEventDestroy(g->stage_ev[b]));
+  }
   if (g->ev0) ignore(hipEventDestroy(g->ev0));
   if (g->ev1) ignore(hipEventDestroy(g->ev1));
   if (g->stream) ignore(hipStreamDestroy(g->stream));
@@ -1515,14 +1519,57 @@ int ent_bfs(ent_graph *g, int32_t root, int32_t *depth_host, ent_stats *st) {
   return ent_copy_to_host(g, g->depth, depth_host, g->n);
 }
 
+// Device-to-host copy on the handle's stream. Pinned destinations (and small
+// copies) take one DMA; pageable ones go through two pinned staging buffers:
+// the DMA of chunk c overlaps the multi-threaded host copy of chunk c - 1
+// (a DMA straight into pageable memory runs at a few GB/s).
+int copy_to_host(ent_graph *g, void *dst, const void *src, size_t bytes) {
+  constexpr size_t kChunk = size_t(8) << 20;
+  hipPointerAttribute_t attr;
+  bool pinned = false;
+  if (hipPointerGetAttributes(&attr, dst) == hipSuccess) pinned = attr.type == hipMemoryTypeHost;
+  else ignore(hipGetLastError());  // pageable memory is unknown to the runtime
+  if (pinned || bytes <= kChunk) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return ENT_OK;
+  }
+  for (int b = 0; b < 2; ++b) {
+    if (!g->stage[b]) HIP_TRY(hipHostMalloc(&g->stage[b], kChunk, hipHostMallocDefault));
+    if (!g->stage_ev[b]) HIP_TRY(hipEventCreateWithFlags(&g->stage_ev[b], hipEventDisableTiming));
+  }
+  const size_t nchunks = (bytes + kChunk - 1) / kChunk;
+  auto chunk_bytes = [&](size_t c) { return std::min(kChunk, bytes - c * kChunk); };
+  auto drain = [&](size_t c) -> int {  // host copy of chunk c out of its staging buffer
+    HIP_TRY(hipEventSynchronize(g->stage_ev[c & 1]));
+    const size_t cb = chunk_bytes(c);
+    char *d = static_cast<char *>(dst) + c * kChunk;
+    const char *sb = static_cast<const char *>(g->stage[c & 1]);
+    parallel_for(int64_t(cb), int64_t(1) << 20, [&](int64_t b0, int64_t e0) {
+      std::memcpy(d + b0, sb + b0, size_t(e0 - b0));
+    });
+    return ENT_OK;
+  };
+  // Buffer c & 1 is free when chunk c is issued: chunk c - 2 was drained in
+  // the previous iteration.
+  for (size_t c = 0; c < nchunks; ++c) {
+    HIP_TRY(hipMemcpyAsync(g->stage[c & 1], static_cast<const char *>(src) + c * kChunk, chunk_bytes(c),
+                           hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipEventRecord(g->stage_ev[c & 1], g->stream));
+    if (c >= 1) {
+      const int rc = drain(c - 1);
+      if (rc) return rc;
+    }
+  }
+  return drain(nchunks - 1);
+}
+
 int ent_copy_to_host(ent_graph *g, const int32_t *src_dev, int32_t *dst_host, int64_t count) {
   clear_error();
   if (!g || !src_dev || !dst_host || count < 0) return set_error(ENT_EINVAL, "bad argument");
   std::lock_guard<std::mutex> lk(g->mu);
   HIP_TRY(hipSetDevice(g->device));
-  HIP_TRY(hipMemcpyAsync(dst_host, src_dev, sizeof(int32_t) * size_t(count), hipMemcpyDeviceToHost, g->stream));
-  HIP_TRY(hipStreamSynchronize(g->stream));
-  return ENT_OK;
+  return copy_to_host(g, dst_host, src_dev, sizeof(int32_t) * size_t(count));
 }
 
 int ent_bfs_gpu_coalescing_mem(const int32_t *src_list, int32_t nsrc, const int64_t *beg_pos,

@@ -62,6 +62,8 @@ struct ent_graph {
   int *depth = nullptr;
   void *aux = nullptr;     // 256 bytes of device scratch for host-side calls (counters)
   int *parent = nullptr;   // parent array of ent_bfs_parents (allocated on first use)
+  void *stage[2] = {nullptr, nullptr};  // pinned staging for copies to pageable host memory
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   unsigned long long *visited = nullptr;
   unsigned long long *front[2] = {nullptr, nullptr};
   unsigned char *mark = nullptr;
