@@ -160,6 +160,11 @@ def bench_plain(args):
         d, st = dg.bfs(int(roots[i]))
         pcie.append(st["reached_edges"] / (time.perf_counter() - t0) / 1e9)
         gpu_depths[i] = d
+    # Schedule-exact examined bytes: the counting kernel variant over the
+    # same roots (outside the timed region; it is slower).
+    dg.configure(count_examined=1)
+    exam = [dg.bfs_device(int(r))["examined_bytes"] for r in roots]
+    dg.configure(count_examined=0)
     used = [(r, dt, st) for (r, dt, st) in per if st["levels"] - 1 > 2]
     gteps = [st["reached_edges"] / dt / 1e9 for (r, dt, st) in used]
     gteps_dev = [st["reached_edges"] / (st["device_ms"] * 1e-3) / 1e9 for (r, dt, st) in used]
@@ -199,7 +204,9 @@ def bench_plain(args):
                      "traffic": traffic.figure(f"plain_rmat{scale}"),
                      "kernel": "whole search (init + level kernels) per BFS, HIP events on "
                                "the handle's stream",
-                     "alg_bytes_per_bfs_mean": int(np.mean(alg_bytes))},
+                     "alg_bytes_per_bfs_mean": int(np.mean(alg_bytes)),
+                     "examined_bytes_per_bfs_mean": int(np.mean(exam)),
+                     "examined_over_alg": round(float(np.mean(exam)) / float(np.mean(alg_bytes)), 3)},
         "cpu_baseline": cpu,
         "extra": {
             "gteps_hmean_device_time": round(hmean(gteps_dev), 3),

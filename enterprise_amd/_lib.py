@@ -40,7 +40,7 @@ class BfsConfig(C.Structure):
     _fields_ = [(name, C.c_int32) for name in (
         "direction", "launch", "alpha", "beta", "t_wave", "t_block", "t_grid",
         "piece", "block_size", "blocks_per_cu", "bu_probe", "bu_chunk_words",
-        "graph_levels", "queue_small", "batch_alpha", "grid_blocks", "hub_cache", "hub_td", "queue_max_degree")]
+        "graph_levels", "queue_small", "batch_alpha", "grid_blocks", "hub_cache", "hub_td", "queue_max_degree", "count_examined")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -52,7 +52,8 @@ class Stats(C.Structure):
                 ("reached_vertices", C.c_int64), ("reached_edges", C.c_int64),
                 ("alg_bytes", C.c_int64), ("device_ms", C.c_double),
                 ("host_ms", C.c_double), ("bu_level_mask", C.c_uint64),
-                ("grid_blocks", C.c_int32), ("block_size", C.c_int32)]
+                ("grid_blocks", C.c_int32), ("block_size", C.c_int32),
+                ("examined_edges", C.c_int64), ("examined_bytes", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

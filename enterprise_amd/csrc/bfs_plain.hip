@@ -96,7 +96,31 @@ struct Control {
   State st[2];
   GridSync sync;    // persistent form only
   Counters cnt[3];  // counts produced by launch i: cnt[i % 3]
+  unsigned long long exam[kStripes * 4];  // counting variant: edges, vertices, sweep bytes by stripe
 };
+
+// Examined-work tally of the counting kernel variant (COUNT): adjacency
+// entries loaded, vertices whose offsets were loaded, and bytes of frontier,
+// visited, mark and depth arrays swept. Compiled out of the default variant.
+template <bool COUNT>
+struct Tally {
+  unsigned long long edges = 0, verts = 0, sweep = 0;
+  __device__ __forceinline__ void e(long long x) {
+    if (COUNT) edges += (unsigned long long)x;
+  }
+  __device__ __forceinline__ void v(long long x) {
+    if (COUNT) verts += (unsigned long long)x;
+  }
+  __device__ __forceinline__ void s(long long x) {
+    if (COUNT) sweep += (unsigned long long)x;
+  }
+};
+
+// Number of i = s + t, s + t + stride, ... below e.
+__device__ __forceinline__ long long strided_count(long long s, long long e, int t, int stride) {
+  const long long span = e - s - t;
+  return span > 0 ? (span + stride - 1) / stride : 0;
+}
 
 struct Args {
   const long long *off;
@@ -297,8 +321,8 @@ __device__ __forceinline__ void td_mark_range_hub(const Args &a, long long s, lo
 // static lists (skipped unless in frontier L); thread- and wavefront-class
 // frontier vertices come from a compacting sweep of front[L & 1]. Every
 // unvisited neighbour gets a byte mark; nothing is counted here.
-template <int BS, bool HUB>
-__device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
+template <int BS, bool HUB, bool COUNT>
+__device__ void td_phase(const Args &a, Smem<BS> &sm, int L, Tally<COUNT> &tl) {
   const unsigned long long *fr = a.front[L & 1];
   const int H = a.hub_mask + 1;
   unsigned char *flags = reinterpret_cast<unsigned char *>(hub_lds + H);
@@ -310,6 +334,7 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
     __syncthreads();
   }
   auto mark_range = [&](long long s, long long e, int t, int stride) {
+    tl.e(strided_count(s, e, t, stride));
     if (HUB) td_mark_range_hub(a, s, e, t, stride, flags);
     else td_mark_range(a, s, e, t, stride);
   };
@@ -318,11 +343,13 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
     if (!bit_of(fr, v)) continue;
     const long long s = a.off[v] + (long long)a.piece_k[p] * a.piece;
     const long long e = min(a.off[v + 1], s + a.piece);
+    if (threadIdx.x == 0) tl.v(1);
     mark_range(s, e, threadIdx.x, BS);
   }
   for (int h = blockIdx.x; h < a.n_hub_block; h += gridDim.x) {
     const int v = a.hub_block[h];
     if (!bit_of(fr, v)) continue;
+    if (threadIdx.x == 0) tl.v(1);
     mark_range(a.off[v], a.off[v + 1], threadIdx.x, BS);
   }
   const int wid = threadIdx.x / kWave, lane = lane_id();
@@ -334,6 +361,7 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
   for (long long ch = xcd_wave_id(); ch < nchunks; ch += nwv) {
     const long long w = ch * CW + lane;
     const unsigned long long word = (lane < CW && w < a.nwords) ? fr[w] : 0ull;
+    if (lane < CW && w < a.nwords) tl.s(8);
     if (__ballot(word != 0) == 0) continue;
     const int total = wave_compact_setup(word, wc, we);
     for (int b0 = 0; b0 < total; b0 += kWave) {
@@ -345,6 +373,7 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
         const int v = int((ch * CW + j) * 64 + bp);
         s = a.off[v];
         e = a.off[v + 1];
+        tl.v(1);
         if (e - s < a.t_wave) mark_range(s, e, 0, 1);
       }
       const long long d = e - s;
@@ -371,8 +400,8 @@ __device__ void td_phase(const Args &a, Smem<BS> &sm, int L) {
 // is vertex 64w + b). New frontier = marked and unvisited. Writes visited,
 // front[(L + 1) & 1] (every word), depth L + 1, clears the marks and counts
 // the new frontier into cnt[slot].
-template <int BS, bool HUB>
-__device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
+template <int BS, bool HUB, bool COUNT>
+__device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<COUNT> &tl) {
   unsigned long long *fn = a.front[(L + 1) & 1];
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
@@ -380,12 +409,15 @@ __device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
   for (long long w = xcd_wave_id(); w < a.nwords; w += nwv) {
     const unsigned long long vis = a.visited[w];
     unsigned long long nw = 0;
+    if (lane == 0) tl.s(8);
     if (~vis) {
       const long long v = w * 64 + lane;
       const bool unvisited = ((vis >> lane) & 1ull) == 0;  // implies v < n
+      if (unvisited) tl.s(1);
       const bool mk = unvisited && a.mark[v] != 0;
       nw = __ballot(mk);
       if (mk) {
+        tl.v(1);
         a.mark[v] = 0;
         a.depth[v] = L + 1;
         const long long d = a.off[v + 1] - a.off[v];
@@ -409,8 +441,8 @@ __device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
 // bu_probe neighbours, hands longer unresolved lists to the whole wavefront
 // (ballot early exit), then writes its words of the next frontier and of the
 // visited bitmap without atomics, the depths, and the counts.
-template <int BS, bool HUB>
-__device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
+template <int BS, bool HUB, bool COUNT>
+__device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<COUNT> &tl) {
   const unsigned long long *fr = a.front[L & 1];
   const int hm = a.hub_mask;
   if (HUB) {  // the table's hubs that are in frontier L, -1 elsewhere
@@ -431,6 +463,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
     const long long w = w0 + lane;
     const bool own = lane < CW && w < a.nwords;
     const unsigned long long vis = own ? a.visited[w] : ~0ull;
+    if (own) tl.s(8);
     wf[lane] = 0;
     const int total = wave_compact_setup(~vis, wc, we);
     for (int b0 = 0; b0 < total; b0 += kWave) {
@@ -446,11 +479,13 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         bitm = 1ull << bp;
         s = a.off[v];
         e = a.off[v + 1];
+        tl.v(1);
         const long long pe = min(e, s + a.bu_probe);
         // Four neighbours per step: their loads overlap; the step stops the
         // scan if any of them is in the frontier.
         for (p = s; p + 3 < pe; p += 4) {
           const int u0 = a.adj[p], u1 = a.adj[p + 1], u2 = a.adj[p + 2], u3 = a.adj[p + 3];
+          tl.e(4);
           // A frontier hub among the four ends the scan from LDS.
           if (HUB && (int(hub_lds[u0 & hm] == u0) | int(hub_lds[u1 & hm] == u1) | int(hub_lds[u2 & hm] == u2) |
                       int(hub_lds[u3 & hm] == u3)) != 0) {
@@ -465,6 +500,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         }
         if (!found) {
           for (; p < pe; ++p) {
+            tl.e(1);
             if (bit_of(fr, a.adj[p])) {
               found = true;
               break;
@@ -493,6 +529,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         bool hit_any = false;
         for (long long q0 = pp; q0 < ee; q0 += 2 * kWave) {
           const long long q = q0 + lane, q2 = q + kWave;
+          tl.e(int(q < ee) + int(q2 < ee));
           bool hit;
           if (HUB) {
             const int x0 = q < ee ? a.adj[q] : -1, x1 = q2 < ee ? a.adj[q2] : -1;
@@ -533,8 +570,8 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
 // t_wave, the whole wavefront) expands a frontier vertex; each unvisited
 // neighbour is claimed with an atomic OR on its visited word, and the
 // claiming lane writes its depth and stages it for the next queue.
-template <int BS>
-__device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
+template <int BS, bool COUNT>
+__device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<COUNT> &tl) {
   const int lane = lane_id();
   if (threadIdx.x < kWave) {
     const int c = int(sm.S.qn[lane] < unsigned(a.qcap_s) ? sm.S.qn[lane] : unsigned(a.qcap_s));
@@ -567,6 +604,7 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
         const long long ij = i + (long long)j * stride;
         u[j] = ij < e ? a.adj[ij] : 0;
         cand[j] = ij < e && !bit_of(a.visited, u[j]);
+        tl.e(ij < e ? 1 : 0);
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j)
@@ -591,6 +629,7 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
       const int v = qin[(long long)lo * a.qcap_s + (k - sm.qpre[lo])];
       s = a.off[v];
       e = a.off[v + 1];
+      tl.v(1);
       if (e - s < a.t_wave) visit_range(s, e, 1);
     }
     unsigned long long wm = __ballot(k < total && e - s >= a.t_wave);
@@ -607,13 +646,14 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot) {
 
 // Rebuilds front[L & 1] from the depth array (depth == L), one wavefront per
 // bitmap word, when a queue-driven level is followed by a bitmap-driven one.
-template <int BS, bool HUB>
-__device__ void rebuild_phase(const Args &a, Smem<BS> &sm, int L) {
+template <int BS, bool HUB, bool COUNT>
+__device__ void rebuild_phase(const Args &a, Smem<BS> &sm, int L, Tally<COUNT> &tl) {
   unsigned long long *fr = a.front[L & 1];
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   for (long long w = xcd_wave_id(); w < a.nwords; w += nwv) {
     const long long v = w * 64 + lane;
+    if (v < a.n) tl.s(4);
     const bool in = v < a.n && a.depth[v] == L;
     const unsigned long long word = __ballot(in);
     if (lane == 0) fr[w] = word;
@@ -803,45 +843,58 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu) {
   }
 }
 
-template <int BS, bool HUB>
+template <int BS, bool HUB, bool COUNT>
 __device__ __forceinline__ void run_phase(const Args &a, Smem<BS> &sm, long long i) {
   const int L = sm.S.level;
   const int slot = int(i % 3);
+  Tally<COUNT> tl;
   switch (sm.S.phase) {
     case PH_TD:
-      if (HUB && a.hub_td) td_phase<BS, HUB>(a, sm, L);
-      else td_phase<BS, false>(a, sm, L);
+      if (HUB && a.hub_td) td_phase<BS, HUB, COUNT>(a, sm, L, tl);
+      else td_phase<BS, false, COUNT>(a, sm, L, tl);
       break;
-    case PH_BU: bu_phase<BS, HUB>(a, sm, L, slot); break;
-    case PH_INSPECT: inspect_phase<BS, HUB>(a, sm, L, slot); break;
-    case PH_TDQ: tdq_phase<BS>(a, sm, L, slot); break;
-    default: rebuild_phase<BS, HUB>(a, sm, L); break;
+    case PH_BU: bu_phase<BS, HUB, COUNT>(a, sm, L, slot, tl); break;
+    case PH_INSPECT: inspect_phase<BS, HUB, COUNT>(a, sm, L, slot, tl); break;
+    case PH_TDQ: tdq_phase<BS, COUNT>(a, sm, L, slot, tl); break;
+    default: rebuild_phase<BS, HUB, COUNT>(a, sm, L, tl); break;
+  }
+  if (COUNT) {  // one atomic per wavefront and counter, on the block's stripe
+    unsigned long long x[3] = {tl.edges, tl.verts, tl.sweep};
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) x[j] += __shfl_xor(x[j], o);
+    if (lane_id() == 0)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (x[j]) atomicAdd(&a.ctl->exam[(blockIdx.x % kStripes) * 4 + j], x[j]);
   }
 }
 
 // Launch i of a search: one phase. No end-of-launch synchronisation: the next
 // launch derives the next state itself.
-// HUB: the variant with the LDS hub table (only launched when it is on).
-template <int BS, bool HUB>
+// HUB: the variant with the LDS hub table; COUNT: the variant that tallies
+// examined work (each only launched when configured).
+template <int BS, bool HUB, bool COUNT>
 __global__ __launch_bounds__(BS) void k_level(Args a, long long i) {
   __shared__ Smem<BS> sm;
   begin_launch(a, sm, i, false);
   if (sm.S.done) return;
-  run_phase<BS, HUB>(a, sm, i);
+  run_phase<BS, HUB, COUNT>(a, sm, i);
 }
 
 // Whole search in one cooperative launch: launches 0, 1, 2, ... separated by
 // grid barriers. Stops on completion, or when a barrier wait exceeds ~4 s in
 // any block (the search is then marked failed). Every exit decision is taken
 // from LDS after a block barrier, so a block leaves the loop as a whole.
-template <int BS, bool HUB>
+template <int BS, bool HUB, bool COUNT>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_persist(Args a) {
   __shared__ Smem<BS> sm;
   __shared__ int stop;
   for (long long i = 0;; ++i) {
     begin_launch(a, sm, i, true);
     if (sm.S.done || sm.S.error) break;
-    run_phase<BS, HUB>(a, sm, i);
+    run_phase<BS, HUB, COUNT>(a, sm, i);
     const bool ok = grid_sync(&a.ctl->sync, 400000000ull);
     if (threadIdx.x == 0) {
       if (!ok) {
@@ -1021,6 +1074,8 @@ int check_config(const ent_bfs_config *c) {
     return set_error(ENT_EINVAL, "hub_cache %d not 0 or in [6, 13]", c->hub_cache);
   if (c->hub_td != 0 && c->hub_td != 1) return set_error(ENT_EINVAL, "hub_td %d not 0 or 1", c->hub_td);
   if (c->queue_max_degree < 0) return set_error(ENT_EINVAL, "queue_max_degree %d", c->queue_max_degree);
+  if (c->count_examined != 0 && c->count_examined != 1)
+    return set_error(ENT_EINVAL, "count_examined %d not 0 or 1", c->count_examined);
   return ENT_OK;
 }
 
@@ -1075,22 +1130,25 @@ struct KernelSet {
   size_t smem;
 };
 
-template <int BS>
-KernelSet kernel_set(bool hub) {
-  return {reinterpret_cast<const void *>(&k_init<BS>),
-          hub ? reinterpret_cast<const void *>(&k_level<BS, true>) : reinterpret_cast<const void *>(&k_level<BS, false>),
-          hub ? reinterpret_cast<const void *>(&k_persist<BS, true>)
-              : reinterpret_cast<const void *>(&k_persist<BS, false>),
-          sizeof(Smem<BS>)};
+template <int BS, bool HUB, bool COUNT>
+KernelSet kernel_set_of() {
+  return {reinterpret_cast<const void *>(&k_init<BS>), reinterpret_cast<const void *>(&k_level<BS, HUB, COUNT>),
+          reinterpret_cast<const void *>(&k_persist<BS, HUB, COUNT>), sizeof(Smem<BS>)};
 }
 
-KernelSet kernels_for(int bs, bool hub) {
+template <int BS>
+KernelSet kernel_set(bool hub, bool count) {
+  if (count) return hub ? kernel_set_of<BS, true, true>() : kernel_set_of<BS, false, true>();
+  return hub ? kernel_set_of<BS, true, false>() : kernel_set_of<BS, false, false>();
+}
+
+KernelSet kernels_for(int bs, bool hub, bool count) {
   switch (bs) {
-    case 64: return kernel_set<64>(hub);
-    case 128: return kernel_set<128>(hub);
-    case 512: return kernel_set<512>(hub);
-    case 1024: return kernel_set<1024>(hub);
-    default: return kernel_set<256>(hub);
+    case 64: return kernel_set<64>(hub, count);
+    case 128: return kernel_set<128>(hub, count);
+    case 512: return kernel_set<512>(hub, count);
+    case 1024: return kernel_set<1024>(hub, count);
+    default: return kernel_set<256>(hub, count);
   }
 }
 
@@ -1153,7 +1211,9 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
   const ent_bfs_config &cfg = g->cfg;
   int rc = build_lists(g);
   if (rc) return rc;
-  const KernelSet ks = kernels_for(cfg.block_size, g->hub_slots > 0);
+  const bool count = cfg.count_examined != 0;
+  const KernelSet ks = kernels_for(cfg.block_size, g->hub_slots > 0, count);
+  if (count) HIP_TRY(hipMemsetAsync(g->ctl->exam, 0, sizeof(g->ctl->exam), g->stream));
   if (ks.smem + hub_lds_bytes(cfg) > 65536)
     return set_error(ENT_EINVAL, "hub_cache %d with block_size %d needs %zu bytes of LDS per block (64 KiB max)",
                      cfg.hub_cache, cfg.block_size, ks.smem + hub_lds_bytes(cfg));
@@ -1319,6 +1379,7 @@ void ent_bfs_default_config(ent_bfs_config *c) {
   c->hub_cache = 0;
   c->hub_td = 0;
   c->queue_max_degree = 0;
+  c->count_examined = 0;
 }
 
 int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int64_t m, int device,
@@ -1509,6 +1570,17 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
     st->bu_level_mask = c.bu_mask;
     st->grid_blocks = g->last_grid;
     st->block_size = g->cfg.block_size;
+    st->examined_bytes = -1;
+    st->examined_edges = -1;
+    if (g->cfg.count_examined) {
+      unsigned long long ex[kStripes * 4];
+      HIP_TRY(hipMemcpy(ex, g->ctl->exam, sizeof ex, hipMemcpyDeviceToHost));
+      unsigned long long t[3] = {0, 0, 0};
+      for (int k = 0; k < kStripes; ++k)
+        for (int j = 0; j < 3; ++j) t[j] += ex[k * 4 + j];
+      st->examined_edges = int64_t(t[0]);
+      st->examined_bytes = int64_t(8 * t[0] + 8 * t[1] + t[2]);
+    }
   }
   return ENT_OK;
 }

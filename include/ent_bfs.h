@@ -88,6 +88,9 @@ typedef struct ent_bfs_config {
   int32_t queue_max_degree; /* a frontier goes through the queue only if all
                                its vertices have degree below this
                                (0 = t_block) */
+  int32_t count_examined;   /* 1: run the kernel variant that counts the
+                               work it examines (ent_stats.examined_*);
+                               for measurement, not speed */
 } ent_bfs_config;
 
 typedef struct ent_stats {
@@ -105,6 +108,14 @@ typedef struct ent_stats {
   uint64_t bu_level_mask;   /* bit L set: level L ran bottom-up (L < 64) */
   int32_t grid_blocks;      /* blocks per level launch / persistent grid */
   int32_t block_size;       /* threads per block */
+  int64_t examined_edges;   /* with count_examined: adjacency entries loaded
+                               (every direction, the early exits included);
+                               else -1 */
+  int64_t examined_bytes;   /* with count_examined: 8 B per adjacency entry
+                               loaded (entry + its status lookup), 8 B per
+                               vertex whose offsets were loaded, and the bytes
+                               of frontier / visited / mark / depth arrays
+                               swept; else -1 */
 } ent_stats;
 
 /* Fills cfg with the library defaults. */

@@ -223,3 +223,29 @@ def test_row_order_upload(oracle):
                 dk.configure(direction=d)
                 _check(oracle, g, dg, int(r), f"ordered/{d}")
                 _check(oracle, g, dk, int(r), f"kept/{d}")
+
+
+def test_examined_counts(oracle, rmat16):
+    """The counting kernel variant leaves depths unchanged. Forced top-down
+    loads every adjacency entry of every reached vertex exactly once, so its
+    examined edges equal the reached-edge count M_c; the early exits of the
+    other schedules examine fewer."""
+    g, dg = rmat16
+    base = dg.config
+    try:
+        for r in g.pick_roots(3, 404):
+            r = int(r)
+            want = oracle.bfs(g.beg_pos, g.adj, r)
+            for launch in (L.LAUNCH_STREAM, L.LAUNCH_PERSISTENT):
+                dg.configure(direction=L.DIR_TOPDOWN, count_examined=1, launch=launch)
+                st = _check(oracle, g, dg, r, f"count/td/{launch}")
+                assert st["examined_edges"] == st["reached_edges"], st
+                assert st["examined_bytes"] >= 8 * st["examined_edges"] + 8 * st["reached_vertices"]
+            dg.configure(direction=L.DIR_AUTO, count_examined=1, launch=L.LAUNCH_STREAM)
+            st = _check(oracle, g, dg, r, "count/auto")
+            assert 0 < st["examined_edges"] < st["reached_edges"], st
+            dg.configure(count_examined=0)
+            got, st0 = dg.bfs(r)
+            assert np.array_equal(got, want) and st0["examined_edges"] == -1
+    finally:
+        dg.set_config(base)
