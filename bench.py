@@ -242,6 +242,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=int, default=160)
     ap.add_argument("--rates", default="100,200,500,1000,2000,5000")
     ap.add_argument("--duration", type=float, default=2.0)
+    ap.add_argument("--plain-max", type=int, default=12,
+                    help="serve: batches of at most this many queries run as single-source searches")
     ap.add_argument("--grid-side", type=int, default=4096)
     ap.add_argument("--grid-forms", default="", help="comma-separated launch forms for --mode grid "
                                                        "(stream,graph,persistent,levels; default all)")

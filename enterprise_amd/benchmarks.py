@@ -227,7 +227,7 @@ def serve(args):
     m_big = gb.m
     del gb  # the host copy is no longer needed
     srv = BFSServer(mem_budget_bytes=int(getattr(args, "serve_budget_gb", 160)) << 30, max_batch=64,
-                    max_levels=28, max_slots=4)
+                    max_levels=28, max_slots=4, plain_max=int(getattr(args, "plain_max", 12)))
     slots_big = srv.add_graph("large", db, share_budget=0.9)
     slots_small = srv.add_graph("small", ds, share_budget=0.1)
     log(f"[serve] slots: large {slots_big}, small {slots_small}")
@@ -237,13 +237,14 @@ def serve(args):
     try:
         for rate in rates:
             for lane in srv.lanes.values():
-                lane.stats.update({"batches": 0, "queries": 0, "reached_edges": 0, "busy_s": 0.0,
-                                   "batch_sizes": []})
+                lane.stats.update({"batches": 0, "plain_batches": 0, "queries": 0, "reached_edges": 0,
+                                   "alg_bytes": 0, "busy_s": 0.0, "batch_sizes": []})
             t1 = time.perf_counter()
             res = run_load(srv, rate, dur, 1005 + rate, roots, {"large": 0.8, "small": 0.2})
             wall = time.perf_counter() - t1
             lat = np.array([q.latency_ms for q in res])
             edges = sum(l.stats["reached_edges"] for l in srv.lanes.values())
+            alg = sum(l.stats["alg_bytes"] for l in srv.lanes.values())
             sizes = sum((l.stats["batch_sizes"] for l in srv.lanes.values()), [])
             # Sampled check against the single-source path.
             rng = np.random.default_rng(rate)
@@ -257,6 +258,7 @@ def serve(args):
             row = {"rate_qps": rate, "queries": len(res), "p50_ms": round(float(np.percentile(lat, 50)), 2),
                    "p99_ms": round(float(np.percentile(lat, 99)), 2),
                    "throughput_qps": round(len(res) / wall, 1), "gteps": round(edges / wall / 1e9, 2),
+                   "alg_GBps": round(alg / wall / 1e9, 1),
                    "mean_batch": round(float(np.mean(sizes)), 1) if sizes else 0,
                    "checked": int(len(sample)), "mismatches": mism}
             log(f"[serve] {row}")
@@ -268,7 +270,8 @@ def serve(args):
     return {"mode": "serve", "metric": "p50/p99 query latency vs GTEPS", "unit": "ms",
             "config": {"workload": f"serving R-MAT {big_scale} (80%) + R-MAT 20, Poisson arrivals, "
                                    "64-source continuous batching", "m_large": int(m_big),
-                       "slots": {"large": slots_big, "small": slots_small}, "duration_s": dur},
+                       "slots": {"large": slots_big, "small": slots_small}, "duration_s": dur,
+                       "plain_max": srv.plain_max},
             "rates": per_rate}
 
 
