@@ -9,17 +9,25 @@
 //   lv    u64[(lcap + 3) * n * W] lv[L][v] = sources whose level-L frontier
 //                                 holds v (new bits only; kept for the output)
 //   jf[2] u64[ceil(n/64)]         joint frontier bitmaps (v in any frontier)
-// One launch per phase, with the same launch-indexed state scheme as the
-// single-source path (launch i reads the state and finished counters of
-// launch i - 1; no fences, no contended atomics):
-//   TD     push: every joint-frontier vertex u ORs its masks into lv[L+1][v]
-//          of its neighbours (minus seen) with fire-and-forget atomics;
-//   APPLY  after TD: seen |= lv[L+1], joint frontier bitmap, counts;
-//   BU     pull: every vertex with unseen bits of still-active sources ORs
-//          the level-L masks of its neighbours, stopping once every such
-//          source has been found (or the list ends);
+// One launch index per phase, with the same launch-indexed state scheme as
+// the single-source path (launch i reads the state and finished counters of
+// launch i - 1; no fences, no contended atomics). The direction is chosen
+// per 64-source word: words whose frontier is large are pulled, lagging ones
+// pushed, and one level may do both:
+//   TD     push: every joint-frontier vertex u ORs the push words of its
+//          masks into lv[L+1][v] of its neighbours (minus seen) with
+//          fire-and-forget atomics;
+//   APPLY  after a TD-only level: seen |= lv[L+1], joint frontier bitmap,
+//          counts;
+//   BU     pull: every vertex with unseen bits of still-active pull-word
+//          sources ORs the level-L masks of its neighbours, stopping once
+//          every such source has been found (or the list ends); the bits
+//          pushed to it at this level are merged in;
 //   OUTPUT replays lv[0..levels) per 64-vertex segment in LDS and writes all
-//          k depth rows with full, coalesced 256-byte stores.
+//          k depth rows with 16-byte streaming stores.
+// The pull runs in its own kernel, launched beside the others for the same
+// launch index, so its mask registers do not cap their occupancy; with
+// ENT_LAUNCH_PERSISTENT the whole search is one cooperative launch instead.
 #include <hip/hip_runtime.h>
 
 #include <chrono>

@@ -17,25 +17,34 @@
 //    ENT_DIR_REFERENCE beside a Beamer alpha/beta rule.
 //
 // MI355X design:
-//  - The frontier of every level is a bitmap (one u64 word per 64 vertices).
-//    Thread- and wavefront-class frontier vertices are found by sweeping that
-//    bitmap with ballot/popcount/DPP compaction; block- and grid-class
-//    vertices (degree >= t_block) come from static per-graph hub lists that
-//    are filtered through the bitmap, so no per-level queue is built.
+//  - The frontier of a level is a bitmap (one u64 word per 64 vertices) or,
+//    for small frontiers, a striped queue (64 stripes). Thread- and
+//    wavefront-class frontier vertices are found by sweeping the bitmap with
+//    ballot/popcount/DPP compaction; block- and grid-class vertices (degree
+//    >= t_block) come from static per-graph hub lists filtered through the
+//    bitmap.
 //  - Top-down writes one byte mark per discovered vertex with plain stores
 //    (no atomics, no returned values on the critical path); an inspection
-//    pass turns marks into the next frontier bitmap, the depth writes and the
-//    frontier counts, one wavefront per bitmap word (coalesced).
+//    pass turns marks into the next frontier bitmap and queue, the depth
+//    writes and the frontier counts, one wavefront per bitmap word.
+//  - Small frontiers expand straight from the queue in one launch per level,
+//    claiming vertices with an atomic OR on their visited word.
 //  - Bottom-up owns whole bitmap words per wavefront, so it updates visited,
 //    next frontier and depth without atomics.
-//  - Counters and the end-of-phase arrival are striped over 64 cache lines so
-//    at most ~32 blocks contend for one address.
+//  - Launch i derives its phase from the state and the finished counters of
+//    launch i - 1 (every block computes the same next state), so launches
+//    need no end-of-phase synchronisation; counters are striped over 64
+//    cache lines so few blocks contend for one address.
+//  - Two kernel variants are launched only when configured: the LDS hub
+//    table (HUB) and the examined-work tally (COUNT). The default variant
+//    carries neither, which keeps its registers and code at their minimum.
 //
 // Device state (HBM, sized at upload):
 //   off int64[n+1], adj int32[m], nzmask u64[W] (degree > 0), W = ceil(n/64)
 //   hub_block int32[] (t_block <= degree < t_grid), piece_v/piece_k int32[]
 //   (degree >= t_grid, split into `piece`-edge pieces)
-//   depth int32[n], visited u64[W], front[2] u64[W], mark u8[n], Control.
+//   depth int32[n], visited u64[W], front[2] u64[W], mark u8[n],
+//   qbuf[2] int32[64 * 4096], Control (states, counters, grid barrier).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,7 +65,7 @@ namespace ent {
 enum Phase : int { PH_TD = 0, PH_BU = 1, PH_INSPECT = 2, PH_TDQ = 3, PH_REBUILD = 4 };
 enum : int { ERR_NONE = 0, ERR_LEVELS = 2, ERR_TIMEOUT = 4 };
 
-constexpr int kStripes = 64;     // counter / arrival stripes
+constexpr int kStripes = 64;     // counter and queue stripes
 constexpr int kStripeU64 = 16;   // 128 B between stripes
 
 // Per-stripe counter words.
