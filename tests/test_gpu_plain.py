@@ -249,3 +249,16 @@ def test_examined_counts(oracle, rmat16):
             assert np.array_equal(got, want) and st0["examined_edges"] == -1
     finally:
         dg.set_config(base)
+
+
+def test_allocation_failure_leaves_handle_usable(oracle, rmat16):
+    """An allocation the device cannot satisfy fails with ENT_ENOMEM and a
+    message; the handle keeps working (searches, multi-source workspace)."""
+    g, dg = rmat16
+    with pytest.raises(L.EntError) as e:
+        dg.device_alloc(1 << 52)  # 4 PiB
+    assert e.value.code == L.ENT_ENOMEM and "hipMalloc" in str(e.value)
+    r = int(g.pick_roots(1, 8)[0])
+    _check(oracle, g, dg, r, "after-enomem")
+    out, _ = dg.bfs_batch(g.pick_roots(8, 9))
+    assert out.shape == (8, g.n)
