@@ -59,6 +59,14 @@ class Stats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class LevelRecord(C.Structure):
+    _fields_ = [("launch", C.c_int32), ("level", C.c_int32), ("phase", C.c_int32), ("pad", C.c_int32),
+                ("frontier_vertices", C.c_int64), ("frontier_edges", C.c_int64)]
+
+
+PHASE_NAMES = {0: "top-down", 1: "bottom-up", 2: "inspect", 3: "top-down queue", 4: "rebuild"}
+
+
 class BatchStats(C.Structure):
     _fields_ = [("k", C.c_int32), ("words", C.c_int32), ("levels", C.c_int32),
                 ("phases", C.c_int32),
@@ -96,6 +104,7 @@ SIGNATURES = [
                                       C.POINTER(C.c_void_p)]),
     ("ent_graph_free", C.c_int, [C.c_void_p]),
     ("ent_graph_download", C.c_int, [C.c_void_p, i64p, i32p]),
+    ("ent_level_trace", C.c_int, [C.c_void_p, C.POINTER(LevelRecord), C.c_int32, C.POINTER(C.c_int32)]),
     ("ent_bfs_parents_device", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("ent_bfs_parents", C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, C.POINTER(Stats)]),
     ("ent_graph_share", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),

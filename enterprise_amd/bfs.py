@@ -106,6 +106,16 @@ class DeviceGraph:
                                    C.c_void_p(depth_dev) if depth_dev else None, C.byref(st)))
         return st.as_dict()
 
+    def level_trace(self):
+        """Per-launch records of the last search run with count_examined=1:
+        list of dicts (launch, level, phase name, frontier vertices and edges)."""
+        recs = (L.LevelRecord * 1024)()
+        cnt = C.c_int32()
+        check(lib().ent_level_trace(self.handle, recs, 1024, C.byref(cnt)))
+        return [{"launch": r.launch, "level": r.level, "phase": L.PHASE_NAMES.get(r.phase, str(r.phase)),
+                 "frontier_vertices": r.frontier_vertices, "frontier_edges": r.frontier_edges}
+                for r in recs[:cnt.value]]
+
     def download(self):
         """(beg_pos int64[n+1], adj int32[m]) as held on the device."""
         b = np.empty(self.n + 1, dtype=np.int64)

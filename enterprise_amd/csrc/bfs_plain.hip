@@ -101,11 +101,19 @@ struct State {
   unsigned int qn[kStripes];  // entries per queue stripe
 };
 
+// Per-launch record of the counting variant (ent_level_trace).
+struct LaunchRec {
+  int launch, level, phase, pad;
+  unsigned long long nf, mf;  // frontier of `level`: vertices, degree sum
+};
+constexpr int kTraceMax = 1024;  // launches recorded per search
+
 struct Control {
   State st[2];
   GridSync sync;    // persistent form only
   Counters cnt[3];  // counts produced by launch i: cnt[i % 3]
   unsigned long long exam[kStripes * 4];  // counting variant: edges, vertices, sweep bytes by stripe
+  LaunchRec trace[kTraceMax];             // counting variant: one record per launch
 };
 
 // Examined-work tally of the counting kernel variant (COUNT): adjacency
@@ -749,7 +757,7 @@ __device__ __forceinline__ void next_state(const Args &a, State &s, const unsign
 // Launch prologue: the first wavefront sums the stripes of the previous
 // launch's counts, thread 0 derives this launch's state into LDS; block 0
 // publishes it and clears the counter slot of launch i + 1.
-template <int BS>
+template <int BS, bool COUNT>
 __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long long i, bool coherent) {
   if (threadIdx.x < kWave) {
     Control *c = a.ctl;
@@ -791,7 +799,22 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
       sm.S.qn[lane] = unsigned(min(qn, (unsigned long long)a.qcap_s));
     wave_sync();
     if (lane == 0) {
+      const int prev_level = sm.S.level;
       next_state(a, sm.S, tot, q_fit);
+      if (COUNT && blockIdx.x == 0 && !sm.S.done && i < kTraceMax) {
+        // The counts of launch i - 1 are the new level's frontier when the
+        // level advanced; a launch that continues a level (inspection,
+        // the phase after a rebuild) repeats the previous record's counts.
+        LaunchRec r;
+        r.launch = int(i);
+        r.level = sm.S.level;
+        r.phase = sm.S.phase;
+        r.pad = 0;
+        const bool same = i > 0 && sm.S.level == prev_level;
+        r.nf = same ? ld_agent(&c->trace[i - 1].nf) : tot[K_NF];
+        r.mf = same ? ld_agent(&c->trace[i - 1].mf) : tot[K_MF];
+        c->trace[i] = r;
+      }
       sm.nf = sm.mf = 0;
       sm.qcount = 0;
       sm.qovf = 0;
@@ -887,7 +910,7 @@ __device__ __forceinline__ void run_phase(const Args &a, Smem<BS> &sm, long long
 template <int BS, bool HUB, bool COUNT>
 __global__ __launch_bounds__(BS) void k_level(Args a, long long i) {
   __shared__ Smem<BS> sm;
-  begin_launch(a, sm, i, false);
+  begin_launch<BS, COUNT>(a, sm, i, false);
   if (sm.S.done) return;
   run_phase<BS, HUB, COUNT>(a, sm, i);
 }
@@ -901,7 +924,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
   __shared__ Smem<BS> sm;
   __shared__ int stop;
   for (long long i = 0;; ++i) {
-    begin_launch(a, sm, i, true);
+    begin_launch<BS, COUNT>(a, sm, i, true);
     if (sm.S.done || sm.S.error) break;
     run_phase<BS, HUB, COUNT>(a, sm, i);
     const bool ok = grid_sync(&a.ctl->sync, 400000000ull);
@@ -1500,6 +1523,28 @@ int ent_graph_download(ent_graph *g, int64_t *beg_pos, int32_t *adj) {
   return ENT_OK;
 }
 
+int ent_level_trace(ent_graph *g, ent_level_record *out, int32_t max, int32_t *count) {
+  clear_error();
+  if (!g || !count || (max > 0 && !out)) return set_error(ENT_EINVAL, "bad argument");
+  std::lock_guard<std::mutex> lk(g->mu);
+  *count = 0;
+  if (g->trace_launches <= 0) return ENT_OK;
+  HIP_TRY(hipSetDevice(g->device));
+  const int k = std::min(g->trace_launches, kTraceMax);
+  std::vector<LaunchRec> rec(size_t(std::max(k, 1)));
+  HIP_TRY(hipMemcpy(rec.data(), g->ctl->trace, sizeof(LaunchRec) * size_t(k), hipMemcpyDeviceToHost));
+  const int nout = std::min(k, int(max));
+  for (int j = 0; j < nout; ++j) {
+    out[j].launch = rec[size_t(j)].launch;
+    out[j].level = rec[size_t(j)].level;
+    out[j].phase = rec[size_t(j)].phase;
+    out[j].frontier_vertices = int64_t(rec[size_t(j)].nf);
+    out[j].frontier_edges = int64_t(rec[size_t(j)].mf);
+  }
+  *count = k;
+  return ENT_OK;
+}
+
 int ent_graph_free(ent_graph *g) {
   clear_error();
   if (!g) return ENT_OK;
@@ -1553,6 +1598,7 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
   int *depth = depth_dev ? depth_dev : g->depth;
   HIP_TRY(hipEventRecord(g->ev0, g->stream));
   State c{};
+  g->trace_launches = 0;
   int rc = run_search(g, root, depth, &c);
   if (rc) {
     ignore(hipStreamSynchronize(g->stream));
@@ -1564,6 +1610,8 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
   if (c.error)
     return set_error(ENT_EINTERNAL, "device reported error flags 0x%x (2 level guard, 4 barrier timeout)", c.error);
   if (!c.done) return set_error(ENT_EINTERNAL, "search ended without completion");
+  // The last counted launch finds the frontier empty and records nothing.
+  if (g->cfg.count_examined) g->trace_launches = c.phases - 1;
   if (st) {
     std::memset(st, 0, sizeof *st);
     st->levels = c.levels;

@@ -62,6 +62,7 @@ struct ent_graph {
   int *depth = nullptr;
   void *aux = nullptr;     // 256 bytes of device scratch for host-side calls (counters)
   int *parent = nullptr;   // parent array of ent_bfs_parents (allocated on first use)
+  int trace_launches = 0;  // launches recorded by the last counting search
   void *stage[2] = {nullptr, nullptr};  // pinned staging for copies to pageable host memory
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   unsigned long long *visited = nullptr;

@@ -155,6 +155,26 @@ int ent_bfs_parents_device(ent_graph *g, int32_t root, const int32_t *depth_dev,
 int ent_bfs_parents(ent_graph *g, int32_t root, int32_t *depth_host,
                     int32_t *parent_host, ent_stats *st);
 
+/* One launch of the last search run with count_examined = 1 (the analogue
+ * of the reference's ENABLE_MONITORING per-level prints,
+ * bfs_gpu_opt.cuh:66-290). phase: 0 top-down, 1 bottom-up, 2 inspection
+ * after top-down, 3 top-down from the queue, 4 frontier rebuild. The
+ * frontier counts are those of `level` when the launch starts. */
+typedef struct ent_level_record {
+  int32_t launch;
+  int32_t level;
+  int32_t phase;
+  int32_t pad;
+  int64_t frontier_vertices;
+  int64_t frontier_edges;
+} ent_level_record;
+
+/* Copies up to max records of the last counting search into out and sets
+ * *count to the number recorded: one per launch that ran a phase, i.e.
+ * stats.phases - 1 (the last launch finds the frontier empty), at most 1024.
+ * *count is 0 after a search without counting. */
+int ent_level_trace(ent_graph *g, ent_level_record *out, int32_t max, int32_t *count);
+
 /* Copies the device CSR back to host arrays (beg_pos: n + 1 entries, adj: m
  * entries; either may be NULL), in the order the device holds it. */
 int ent_graph_download(ent_graph *g, int64_t *beg_pos, int32_t *adj);

@@ -262,3 +262,37 @@ def test_allocation_failure_leaves_handle_usable(oracle, rmat16):
     _check(oracle, g, dg, r, "after-enomem")
     out, _ = dg.bfs_batch(g.pick_roots(8, 9))
     assert out.shape == (8, g.n)
+
+
+def test_level_trace(oracle, rmat16):
+    """The counting variant records every launch of the search: levels never
+    decrease, and each record's frontier vertex and degree counts equal the
+    oracle's for its level. Without counting there are no records."""
+    g, dg = rmat16
+    base = dg.config
+    deg = np.diff(g.beg_pos)
+    try:
+        for launch in (L.LAUNCH_STREAM, L.LAUNCH_PERSISTENT):
+            for d in (L.DIR_AUTO, L.DIR_TOPDOWN):
+                dg.configure(count_examined=1, launch=launch, direction=d)
+                r = int(g.pick_roots(1, 55)[0])
+                st = _check(oracle, g, dg, r, f"trace/{launch}/{d}")
+                recs = dg.level_trace()
+                lv = [(x["level"], x["phase"]) for x in recs]
+                # Every launch but the last, which finds the frontier empty.
+                assert len(recs) == st["phases"] - 1, (lv, st)
+                assert [x["launch"] for x in recs] == list(range(len(recs)))
+                lv = [x["level"] for x in recs]
+                assert lv == sorted(lv) and lv[0] == 0 and lv[-1] == st["levels"] - 1, lv
+                want = oracle.bfs(g.beg_pos, g.adj, r)
+                for x in recs:
+                    at = want == x["level"]
+                    assert x["frontier_vertices"] == int(at.sum()), x
+                    assert x["frontier_edges"] == int(deg[at].sum()), x
+                if d == L.DIR_TOPDOWN:
+                    assert "bottom-up" not in {x["phase"] for x in recs}
+        dg.configure(count_examined=0)
+        dg.bfs(int(g.pick_roots(1, 56)[0]), copy=False)
+        assert dg.level_trace() == []
+    finally:
+        dg.set_config(base)
