@@ -762,6 +762,15 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
   if (threadIdx.x < kWave) {
     Control *c = a.ctl;
     const int lane = threadIdx.x;
+    // The previous state (written by block 0 of launch i - 1, or by another
+    // block inside a persistent launch) and the previous launch's counts are
+    // loaded together: one memory round trip for the whole prologue.
+    constexpr int kStateWords = int(sizeof(State) / 4);
+    static_assert(kStateWords > kWave && kStateWords <= 2 * kWave, "two state words per lane");
+    const unsigned *s32 = reinterpret_cast<const unsigned *>(&c->st[(i + 1) & 1]);
+    const unsigned sw0 = coherent ? ld_agent(s32 + lane) : s32[lane];
+    unsigned sw1 = 0;
+    if (lane + kWave < kStateWords) sw1 = coherent ? ld_agent(s32 + lane + kWave) : s32[lane + kWave];
     const Counters &pc = c->cnt[(i + 2) % 3];
     const unsigned long long *src = &pc.v[lane * kStripeU64];
     unsigned long long nf = coherent ? ld_agent(src + K_NF) : src[K_NF];
@@ -784,12 +793,10 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
       tot[K_QOVF] = ovf;
     }
     const bool q_fit = __ballot(qn > (unsigned long long)a.qcap_s) == 0;
-    // The previous state (written by block 0 of launch i - 1, or by another
-    // block inside a persistent launch) is copied word by word into LDS.
     {
-      const unsigned *s32 = reinterpret_cast<const unsigned *>(&c->st[(i + 1) & 1]);
       unsigned *d32 = reinterpret_cast<unsigned *>(&sm.S);
-      for (int w = lane; w < int(sizeof(State) / 4); w += kWave) d32[w] = coherent ? ld_agent(s32 + w) : s32[w];
+      d32[lane] = sw0;
+      if (lane + kWave < kStateWords) d32[lane + kWave] = sw1;
     }
     wave_sync();
     // Queue stripe sizes of the frontier just produced; only meaningful when
