@@ -601,34 +601,38 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<CO
   const int *qin = a.qbuf[L & 1];
   unsigned long long nf = 0, mf = 0;
   // Claims the unvisited among adj[i], i = i0, i0 + stride, ... < e, four
-  // edges per step: visited bits are tested first, then the atomics of the
-  // step are issued together so their return latencies overlap.
-  auto claim = [&](int u, unsigned long long old) {
-    if (old & (1ull << (u & 63))) return;
-    a.depth[u] = L + 1;
-    const long long d = a.off[u + 1] - a.off[u];
-    nf += 1;
-    mf += (unsigned long long)d;
-    stage_push(sm, u, d);
-  };
+  // edges per step. The levels expanded here are short latency chains, so
+  // each neighbour goes straight to the claiming atomic (no visited-bit load
+  // first) and its row offsets are loaded alongside: adjacency, then atomics
+  // and offsets together, two memory round trips per step.
   auto visit_range = [&](long long i0, long long e, int stride) {
     for (long long i = i0; i < e; i += 4LL * stride) {
       int u[4];
-      bool cand[4];
+      bool in[4];
       unsigned long long old[4];
+      long long us[4], ue[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const long long ij = i + (long long)j * stride;
-        u[j] = ij < e ? a.adj[ij] : 0;
-        cand[j] = ij < e && !bit_of(a.visited, u[j]);
-        tl.e(ij < e ? 1 : 0);
+        in[j] = ij < e;
+        u[j] = in[j] ? a.adj[ij] : 0;
+        tl.e(in[j] ? 1 : 0);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        old[j] = in[j] ? atomicOr(a.visited + (u[j] >> 6), 1ull << (u[j] & 63)) : ~0ull;
+        us[j] = in[j] ? a.off[u[j]] : 0;
+        ue[j] = in[j] ? a.off[u[j] + 1] : 0;
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        old[j] = cand[j] ? atomicOr(a.visited + (u[j] >> 6), 1ull << (u[j] & 63)) : ~0ull;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (cand[j]) claim(u[j], old[j]);
+        if (!(old[j] & (1ull << (u[j] & 63)))) {
+          const long long d = ue[j] - us[j];
+          a.depth[u[j]] = L + 1;
+          nf += 1;
+          mf += (unsigned long long)d;
+          stage_push(sm, u[j], d);
+        }
     }
   };
   const long long gt = (long long)blockIdx.x * BS + threadIdx.x;
