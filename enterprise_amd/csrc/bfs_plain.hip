@@ -635,9 +635,12 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<CO
         }
     }
   };
-  const long long gt = (long long)blockIdx.x * BS + threadIdx.x;
+  // 64-entry chunks go to wavefronts of consecutive blocks (wavefront j of
+  // every block before wavefront j + 1 of any): a frontier of a few thousand
+  // vertices spreads over that many CUs instead of filling a few blocks.
+  const long long wv = (long long)(threadIdx.x / kWave) * gridDim.x + blockIdx.x;
   const long long nt = (long long)gridDim.x * BS;
-  for (long long base = gt - lane; base < total; base += nt) {
+  for (long long base = wv * kWave; base < total; base += nt) {
     const long long k = base + lane;
     long long s = 0, e = 0;
     if (k < total) {
