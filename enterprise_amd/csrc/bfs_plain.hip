@@ -199,13 +199,17 @@ __device__ __forceinline__ bool bit_of(const unsigned long long *bm, int v) {
   return ((bm[v >> 6] >> (v & 63)) & 1ull) != 0;
 }
 
-// Stages a newly found frontier vertex for the block's queue flush.
+// Stages a newly found frontier vertex for the block's queue flush (the
+// caller folds its degree into the lane's running maximum, see deg_max).
 template <int BS>
-__device__ __forceinline__ void stage_push(Smem<BS> &sm, int v, long long deg) {
+__device__ __forceinline__ void stage_push(Smem<BS> &sm, int v) {
   const int idx = atomicAdd(&sm.qcount, 1);
   if (idx < kQStage) sm.qstage[idx] = v;
   else sm.qovf = 1;
-  atomicMax(&sm.maxdeg, int(min(deg, (long long)INT32_MAX)));
+}
+
+__device__ __forceinline__ unsigned deg_max(unsigned md, long long deg) {
+  return max(md, unsigned(min(deg, (long long)INT32_MAX)));
 }
 
 // Adds a block's frontier counts to its stripe and appends its staged queue
@@ -235,16 +239,20 @@ __device__ __forceinline__ void add_counts(const Args &a, Smem<BS> &sm, int slot
 }
 
 // Sums a per-lane count over the wavefront and adds it to the block total.
+// Also folds the lanes' largest frontier degree md into the block maximum.
 template <int BS>
-__device__ __forceinline__ void wave_add_counts(Smem<BS> &sm, unsigned long long nf, unsigned long long mf) {
+__device__ __forceinline__ void wave_add_counts(Smem<BS> &sm, unsigned long long nf, unsigned long long mf,
+                                                unsigned md) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     nf += __shfl_xor(nf, o);
     mf += __shfl_xor(mf, o);
+    md = max(md, unsigned(__shfl_xor(int(md), o)));
   }
   if (lane_id() == 0 && nf) {
     atomicAdd(&sm.nf, nf);
     atomicAdd(&sm.mf, mf);
+    atomicMax(&sm.maxdeg, int(md));
   }
 }
 
@@ -423,6 +431,7 @@ __device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tall
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long nf = 0, mf = 0;
+  unsigned md = 0;
   for (long long w = xcd_wave_id(); w < a.nwords; w += nwv) {
     const unsigned long long vis = a.visited[w];
     unsigned long long nw = 0;
@@ -440,14 +449,15 @@ __device__ void inspect_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tall
         const long long d = a.off[v + 1] - a.off[v];
         nf += 1;
         mf += (unsigned long long)d;
-        stage_push(sm, int(v), d);
+        md = deg_max(md, d);
+        stage_push(sm, int(v));
         hub_note<HUB>(a, int(v), d, L + 1);
       }
       if (lane == 0 && nw) a.visited[w] = vis | nw;
     }
     if (lane == 0) fn[w] = nw;
   }
-  wave_add_counts(sm, nf, mf);
+  wave_add_counts(sm, nf, mf, md);
   add_counts(a, sm, slot, (L + 1) & 1);
 }
 
@@ -475,6 +485,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<COU
   const long long nchunks = ((long long)a.nwords + CW - 1) / CW;
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long nf = 0, mf = 0;
+  unsigned md = 0;
   for (long long ch = xcd_wave_id(); ch < nchunks; ch += nwv) {
     const long long w0 = ch * CW;
     const long long w = w0 + lane;
@@ -530,7 +541,8 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<COU
         a.depth[v] = L + 1;
         nf += 1;
         mf += (unsigned long long)(e - s);
-        stage_push(sm, v, e - s);
+        md = deg_max(md, e - s);
+        stage_push(sm, v);
         hub_note<HUB>(a, v, e - s, L + 1);
       }
       unsigned long long lmask = __ballot(k < total && !found && p < e);
@@ -565,7 +577,8 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<COU
           a.depth[vv] = L + 1;
           nf += 1;
           mf += (unsigned long long)(ee - ss);
-          stage_push(sm, vv, ee - ss);
+          md = deg_max(md, ee - ss);
+          stage_push(sm, vv);
           hub_note<HUB>(a, vv, ee - ss, L + 1);
         }
       }
@@ -578,7 +591,7 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<COU
     }
     wave_sync();
   }
-  wave_add_counts(sm, nf, mf);
+  wave_add_counts(sm, nf, mf, md);
   add_counts(a, sm, slot, (L + 1) & 1);
 }
 
@@ -600,6 +613,7 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<CO
   const int total = sm.qpre[kStripes];
   const int *qin = a.qbuf[L & 1];
   unsigned long long nf = 0, mf = 0;
+  unsigned md = 0;
   // Claims the unvisited among adj[i], i = i0, i0 + stride, ... < e, four
   // edges per step. The levels expanded here are short latency chains, so
   // each neighbour goes straight to the claiming atomic (no visited-bit load
@@ -624,14 +638,20 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<CO
         us[j] = in[j] ? a.off[u[j]] : 0;
         ue[j] = in[j] ? a.off[u[j] + 1] : 0;
       }
+      // Every atomic result is read before the first depth store: a wait
+      // for a result issued behind a store would wait for the store too.
+      bool won[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) won[j] = !(old[j] & (1ull << (u[j] & 63)));
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (!(old[j] & (1ull << (u[j] & 63)))) {
+        if (won[j]) {
           const long long d = ue[j] - us[j];
           a.depth[u[j]] = L + 1;
           nf += 1;
           mf += (unsigned long long)d;
-          stage_push(sm, u[j], d);
+          md = deg_max(md, d);
+          stage_push(sm, u[j]);
         }
     }
   };
@@ -664,7 +684,7 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<CO
       visit_range(ss + lane, ee, kWave);
     }
   }
-  wave_add_counts(sm, nf, mf);
+  wave_add_counts(sm, nf, mf, md);
   add_counts(a, sm, slot, (L + 1) & 1);
 }
 
