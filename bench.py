@@ -10,8 +10,9 @@ harmonic mean of per-root GTEPS with the reference's numerator M_c = sum of
 degrees over reached vertices (validate.h:115-123).
 
 Other modes (--mode): grid (4096^2 high-diameter grid, per-level cost of the
-launch forms), batch (512-source multi-source BFS on R-MAT scale 23) and serve
-(continuous-batching serving loop); each prints its own JSON line.
+launch forms), batch (512-source multi-source BFS on R-MAT scale 23), serve
+(continuous-batching serving loop) and csr (CSR construction on the device
+from the raw tuples of R-MAT scale --scale); each prints its own JSON line.
 """
 import argparse
 import json
@@ -232,7 +233,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=8)
-    ap.add_argument("--mode", default="plain", choices=["plain", "grid", "batch", "serve", "calib"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "grid", "batch", "serve", "calib", "csr"])
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--launch", default=None, choices=["persistent", "levels", "graph", "stream"])
     ap.add_argument("--no-cpu", action="store_true")

@@ -1,5 +1,5 @@
-"""ctypes binding of libent_bfs.so (the C ABI declared in include/ent_bfs.h and
-include/ent_graph.h). No torch types cross the boundary: plain pointers and
+"""ctypes binding of libent_bfs.so (the C ABI declared in include/ent_bfs.h,
+include/ent_graph.h, include/ent_batch.h and include/ent_convert.h). No torch types cross the boundary: plain pointers and
 sizes only.
 
 The library is loaded from enterprise_amd/lib/ (built in-tree by
@@ -26,6 +26,8 @@ ENT_ECANCELED = -7
 DIR_AUTO, DIR_REFERENCE, DIR_TOPDOWN, DIR_BOTTOMUP = 0, 1, 2, 3
 LAUNCH_PERSISTENT, LAUNCH_LEVELS, LAUNCH_GRAPH, LAUNCH_STREAM = 0, 1, 2, 3
 UPLOAD_KEEP_ORDER = 1
+CSR_REVERSE, CSR_NO_LOOPS, CSR_SORT, CSR_DEDUP = 1, 2, 4, 8
+CSR_SIMPLE = CSR_REVERSE | CSR_NO_LOOPS | CSR_DEDUP
 
 i32p = C.POINTER(C.c_int32)
 i64p = C.POINTER(C.c_int64)
@@ -34,6 +36,19 @@ i64p = C.POINTER(C.c_int64)
 class CSR(C.Structure):
     _fields_ = [("n", C.c_int64), ("m", C.c_int64),
                 ("beg_pos", i64p), ("adj", i32p)]
+
+
+class Tuples(C.Structure):
+    _fields_ = [("n", C.c_int64), ("ne", C.c_int64), ("vmin", C.c_int64),
+                ("src", i32p), ("dst", i32p)]
+
+
+class BuildStats(C.Structure):
+    _fields_ = [("records", C.c_int64), ("m", C.c_int64), ("key_bits", C.c_int32),
+                ("pad", C.c_int32), ("device_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
 
 
 class BfsConfig(C.Structure):
@@ -129,6 +144,18 @@ SIGNATURES = [
     ("ent_device_free", C.c_int, [C.c_void_p, C.c_void_p]),
     ("ent_device_row_hash", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
                                       C.POINTER(C.c_uint64)]),
+    ("ent_tuples_read", C.c_int, [C.c_char_p, C.c_int64, C.POINTER(Tuples)]),
+    ("ent_tuples_free", None, [C.POINTER(Tuples)]),
+    ("ent_gen_rmat_tuples", C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                      C.c_uint64, C.c_int32, i32p, i32p]),
+    ("ent_csr_build", C.c_int, [C.c_int32, C.c_int64, C.c_int64, i32p, i32p, i32p, C.c_int32,
+                                C.POINTER(CSR), C.POINTER(i32p), C.POINTER(BuildStats)]),
+    ("ent_csr_build_device", C.c_int, [C.c_int32, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p,
+                                       C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                                       C.POINTER(C.c_int64), C.POINTER(BuildStats)]),
+    ("ent_graph_upload_edges", C.c_int, [i32p, i32p, C.c_int64, C.c_int64, C.c_int32, C.c_int32,
+                                         C.c_int32, C.POINTER(C.c_void_p)]),
+    ("ent_csr_write_ref_files", C.c_int, [C.POINTER(CSR), i32p, C.c_char_p]),
 ]
 
 _lib = None

@@ -48,6 +48,23 @@ class DeviceGraph:
                                         L.UPLOAD_KEEP_ORDER if keep_order else 0, C.byref(h)))
         self._h = h
 
+    @classmethod
+    def from_edges(cls, n, src, dst, flags=L.CSR_SIMPLE, device=0, keep_order=False):
+        """Handle whose CSR is built on the device from tuples (ent_convert.h)."""
+        s = np.ascontiguousarray(src, dtype=np.int32)
+        d = np.ascontiguousarray(dst, dtype=np.int32)
+        if s.shape != d.shape:
+            raise ValueError("src and dst differ in length")
+        h = C.c_void_p()
+        check(lib().ent_graph_upload_edges(s.ctypes.data_as(i32p), d.ctypes.data_as(i32p), int(n), int(s.size),
+                                           int(flags), int(device), L.UPLOAD_KEEP_ORDER if keep_order else 0,
+                                           C.byref(h)))
+        g = cls.__new__(cls)
+        g._h = h
+        info = g.info()
+        g.n, g.m, g.device = info["n"], info["m"], device
+        return g
+
     @property
     def handle(self):
         if not self._h:

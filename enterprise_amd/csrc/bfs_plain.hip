@@ -1456,6 +1456,15 @@ int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int6
 int ent_graph_upload_ex(const int64_t *beg_pos, const int32_t *adj, int64_t n, int64_t m, int device,
                         int32_t flags, ent_graph **out) {
   clear_error();
+  return ent::upload_graph(beg_pos, adj, false, n, m, device, flags, out);
+}
+
+}  // extern "C"
+
+namespace ent {
+
+int upload_graph(const int64_t *beg_pos, const int32_t *adj, bool adj_on_device, int64_t n, int64_t m, int device,
+                 int32_t flags, ent_graph **out) {
   if (flags & ~int32_t(ENT_UPLOAD_KEEP_ORDER)) return set_error(ENT_EINVAL, "unknown upload flags 0x%x", flags);
   if (!out) return set_error(ENT_EINVAL, "out is NULL");
   *out = nullptr;
@@ -1464,12 +1473,14 @@ int ent_graph_upload_ex(const int64_t *beg_pos, const int32_t *adj, int64_t n, i
   if (m < 0 || beg_pos[0] != 0 || beg_pos[n] != m)
     return set_error(ENT_EINVAL, "offsets do not span [0, m=%lld]", (long long)m);
   if (device < 0) return set_error(ENT_EINVAL, "device %d", device);
-  // Host-side structural checks: monotone offsets, degrees < 2^31, in-range ids.
+  // Host-side structural checks: monotone offsets, degrees < 2^31, in-range
+  // ids (ids of a device-built adjacency come from checked tuples).
   {
     std::atomic<int64_t> bad{-1};
     parallel_for(n, 1 << 15, [&](int64_t b, int64_t e) {
       for (int64_t v = b; v < e; ++v) {
         if (beg_pos[v + 1] < beg_pos[v] || beg_pos[v + 1] - beg_pos[v] > INT32_MAX) { bad.store(v); return; }
+        if (adj_on_device) continue;
         for (int64_t i = beg_pos[v]; i < beg_pos[v + 1]; ++i)
           if (adj[i] < 0 || adj[i] >= n) { bad.store(v); return; }
       }
@@ -1530,7 +1541,8 @@ int ent_graph_upload_ex(const int64_t *beg_pos, const int32_t *adj, int64_t n, i
   g->adj = g->store->adj;
   g->nzmask = g->store->nzmask;
   UP_TRY(hipMemcpy(g->off, beg_pos, sizeof(long long) * size_t(n + 1), hipMemcpyHostToDevice));
-  if (m) UP_TRY(hipMemcpy(g->adj, adj, sizeof(int) * size_t(m), hipMemcpyHostToDevice));
+  if (m)
+    UP_TRY(hipMemcpy(g->adj, adj, sizeof(int) * size_t(m), adj_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   UP_TRY(hipMemcpy(g->nzmask, nz.data(), sizeof(unsigned long long) * size_t(g->nwords), hipMemcpyHostToDevice));
   rc = alloc_workspace(g);
   if (!rc && !(flags & ENT_UPLOAD_KEEP_ORDER)) rc = order_rows_by_degree(g);
@@ -1543,6 +1555,10 @@ int ent_graph_upload_ex(const int64_t *beg_pos, const int32_t *adj, int64_t n, i
   *out = g;
   return ENT_OK;
 }
+
+}  // namespace ent
+
+extern "C" {
 
 int ent_graph_download(ent_graph *g, int64_t *beg_pos, int32_t *adj) {
   clear_error();

@@ -39,6 +39,10 @@ struct GraphStore {
 // Reorders every adjacency row by descending floor(log2(neighbour degree)),
 // stably (adj_order.hip).
 int order_rows_by_degree(ent_graph *g);
+// ent_graph_upload_ex with the adjacency either in host memory or already in
+// device memory of `device` (adj_on_device; copied device to device).
+int upload_graph(const int64_t *beg_pos, const int32_t *adj, bool adj_on_device, int64_t n, int64_t m, int device,
+                 int32_t flags, ent_graph **out);
 }  // namespace ent
 
 struct ent_graph {

@@ -5,6 +5,11 @@
 // tuple_text_to_bin.cpp (int64 _beg_pos.bin / _csr.bin, :153-184) and
 // narrows them to int32 (graph.hpp:60-66, :80-87). Its root picker is
 // main.cu:55-78.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cerrno>
 #include <cmath>
@@ -14,6 +19,7 @@
 #include <vector>
 
 #include "ent_common.h"
+#include "ent_convert.h"
 
 namespace ent {
 
@@ -124,6 +130,52 @@ int finish_csr(int64_t n, const int64_t *start, int32_t *raw, ent_csr *out) {
   return ENT_OK;
 }
 
+// R-MAT tuple i (Graph500 Kronecker recursion without noise) from a
+// counter-based RNG, so tuples do not depend on the thread count; ids
+// optionally permuted by a seeded Fisher-Yates shuffle.
+struct Rmat {
+  int32_t scale;
+  uint64_t seed, ta, tb, tc;
+  std::vector<int32_t> perm;
+  Rmat(int32_t scale_, double a, double b, double c, uint64_t seed_, int32_t relabel) : scale(scale_), seed(seed_) {
+    // Integer thresholds on a 32-bit uniform draw: quadrant 0 (a), 1 (b:
+    // column bit set), 2 (c: row bit set), 3 (d: both bits set).
+    ta = uint64_t(std::ldexp(a, 32));
+    tb = uint64_t(std::ldexp(a + b, 32));
+    tc = uint64_t(std::ldexp(a + b + c, 32));
+    if (relabel) {
+      const int64_t n = int64_t(1) << scale;
+      perm.resize(size_t(n));
+      std::iota(perm.begin(), perm.end(), 0);
+      uint64_t st = mix64(seed ^ 0xA5A5A5A55A5A5A5Aull);
+      for (int64_t i = n - 1; i > 0; --i) {
+        uint64_t j = splitmix_next(st) % uint64_t(i + 1);
+        std::swap(perm[size_t(i)], perm[size_t(j)]);
+      }
+    }
+  }
+  void tuple(int64_t i, int64_t &u, int64_t &v) const {
+    uint64_t st = hash2(seed, uint64_t(i));
+    int64_t r = 0, col = 0;
+    for (int32_t l = 0; l < scale; ++l) {
+      uint64_t x = splitmix_next(st) >> 32;
+      int rb = 0, cb = 0;
+      if (x < ta) {
+      } else if (x < tb) {
+        cb = 1;
+      } else if (x < tc) {
+        rb = 1;
+      } else {
+        rb = cb = 1;
+      }
+      r = (r << 1) | rb;
+      col = (col << 1) | cb;
+    }
+    u = perm.empty() ? r : perm[size_t(r)];
+    v = perm.empty() ? col : perm[size_t(col)];
+  }
+};
+
 // Bucket fill shared by R-MAT and edge-list construction: edge(i, &u, &v)
 // returns false for tuples to skip. Each kept tuple is stored both ways.
 template <class EdgeFn>
@@ -212,45 +264,34 @@ int ent_gen_rmat(int32_t scale, int32_t edgefactor, double a, double b, double c
     return set_error(ENT_EINVAL, "bad R-MAT probabilities a=%g b=%g c=%g", a, b, c);
   const int64_t n = int64_t(1) << scale;
   const int64_t ne = n * edgefactor;
-  // Integer thresholds on a 32-bit uniform draw: quadrant 0 (a), 1 (b: column
-  // bit set), 2 (c: row bit set), 3 (d: both bits set).
-  const uint64_t ta = uint64_t(std::ldexp(a, 32));
-  const uint64_t tb = uint64_t(std::ldexp(a + b, 32));
-  const uint64_t tc = uint64_t(std::ldexp(a + b + c, 32));
-  std::vector<int32_t> perm;
-  if (relabel) {
-    perm.resize(size_t(n));
-    std::iota(perm.begin(), perm.end(), 0);
-    uint64_t st = mix64(seed ^ 0xA5A5A5A55A5A5A5Aull);
-    for (int64_t i = n - 1; i > 0; --i) {
-      uint64_t j = splitmix_next(st) % uint64_t(i + 1);
-      std::swap(perm[size_t(i)], perm[size_t(j)]);
-    }
-  }
-  const int32_t *pm = relabel ? perm.data() : nullptr;
+  const Rmat gen(scale, a, b, c, seed, relabel);
   auto edge = [&](int64_t i, int64_t &u, int64_t &v) -> bool {
-    uint64_t st = hash2(seed, uint64_t(i));
-    int64_t r = 0, col = 0;
-    for (int32_t l = 0; l < scale; ++l) {
-      uint64_t x = splitmix_next(st) >> 32;
-      int rb = 0, cb = 0;
-      if (x < ta) {
-      } else if (x < tb) {
-        cb = 1;
-      } else if (x < tc) {
-        rb = 1;
-      } else {
-        rb = cb = 1;
-      }
-      r = (r << 1) | rb;
-      col = (col << 1) | cb;
-    }
-    if (r == col) return false;
-    u = pm ? pm[r] : r;
-    v = pm ? pm[col] : col;
-    return true;
+    gen.tuple(i, u, v);
+    return u != v;
   };
   return build_symmetric(n, ne, edge, out);
+}
+
+int ent_gen_rmat_tuples(int32_t scale, int32_t edgefactor, double a, double b, double c, uint64_t seed,
+                        int32_t relabel, int32_t *src, int32_t *dst) {
+  clear_error();
+  if (!src || !dst) return set_error(ENT_EINVAL, "NULL output");
+  if (scale < 1 || scale > 30) return set_error(ENT_EINVAL, "scale %d outside [1, 30]", scale);
+  if (edgefactor < 1 || edgefactor > 1024)
+    return set_error(ENT_EINVAL, "edgefactor %d outside [1, 1024]", edgefactor);
+  if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0))
+    return set_error(ENT_EINVAL, "bad R-MAT probabilities a=%g b=%g c=%g", a, b, c);
+  const int64_t ne = (int64_t(1) << scale) * edgefactor;
+  const Rmat gen(scale, a, b, c, seed, relabel);
+  parallel_for(ne, 1 << 16, [&](int64_t bgn, int64_t end) {
+    for (int64_t i = bgn; i < end; ++i) {
+      int64_t u, v;
+      gen.tuple(i, u, v);
+      src[i] = int32_t(u);
+      dst[i] = int32_t(v);
+    }
+  });
+  return ENT_OK;
 }
 
 int ent_csr_from_edges(int64_t n, int64_t ne, const int32_t *src, const int32_t *dst,
@@ -615,6 +656,179 @@ int ent_pick_roots_mixed(const ent_csr *g, int32_t k, int32_t k_hub, uint64_t se
     std::swap(low[size_t(i)], low[j]);
     roots[k_hub + i] = low[size_t(i)];
   }
+  return ENT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Text tuple files (the reference converter's input) and its output files.
+
+namespace ent {
+namespace {
+
+inline bool tuple_sep(char ch) { return ch == ' ' || ch == '\n' || ch == '\t'; }
+
+// atol on [p, end): leading white space, optional sign, decimal digits.
+// Like atol it may read past the end of the token when the token itself
+// starts with other white space ('\r', '\v', '\f').
+int64_t text_atol(const char *p, const char *end) {
+  while (p < end && (*p == ' ' || (*p >= '\t' && *p <= '\r'))) ++p;
+  bool neg = false;
+  if (p < end && (*p == '+' || *p == '-')) neg = *p++ == '-';
+  uint64_t x = 0;
+  while (p < end && *p >= '0' && *p <= '9') x = x * 10 + uint64_t(*p++ - '0');
+  return neg ? -int64_t(x) : int64_t(x);
+}
+
+// Token starts of the reference's scan (tuple_text_to_bin.cpp:112-139):
+// offset 0, and every non-separator that follows a separator.
+inline bool token_start(const char *d, int64_t p) { return p == 0 || (!tuple_sep(d[p]) && tuple_sep(d[p - 1])); }
+
+struct MappedFile {
+  const char *data = nullptr;
+  size_t size = 0;
+  int fd = -1;
+  ~MappedFile() {
+    if (data && size) munmap(const_cast<char *>(data), size);
+    if (fd >= 0) close(fd);
+  }
+};
+
+}  // namespace
+}  // namespace ent
+
+extern "C" {
+
+void ent_tuples_free(ent_tuples *t) {
+  if (!t) return;
+  std::free(t->src);
+  std::free(t->dst);
+  t->src = t->dst = nullptr;
+  t->n = t->ne = t->vmin = 0;
+}
+
+int ent_tuples_read(const char *path, int64_t skip_lines, ent_tuples *out) {
+  clear_error();
+  if (!path || !out) return set_error(ENT_EINVAL, "NULL argument");
+  if (skip_lines < 0) return set_error(ENT_EINVAL, "skip_lines %lld < 0", (long long)skip_lines);
+  std::memset(out, 0, sizeof *out);
+  MappedFile f;
+  f.fd = open(path, O_RDONLY);
+  if (f.fd < 0) return set_error(ENT_EIO, "cannot open %s: %s", path, std::strerror(errno));
+  struct stat sb;
+  if (fstat(f.fd, &sb) != 0) return set_error(ENT_EIO, "cannot stat %s: %s", path, std::strerror(errno));
+  f.size = size_t(sb.st_size);
+  if (f.size) {
+    void *p = mmap(nullptr, f.size, PROT_READ, MAP_PRIVATE, f.fd, 0);
+    if (p == MAP_FAILED) return set_error(ENT_EIO, "cannot map %s: %s", path, std::strerror(errno));
+    f.data = static_cast<const char *>(p);
+  }
+  const int64_t size = int64_t(f.size);
+  // Line skipping as tuple_text_to_bin.cpp:80-97 does it: the offset advances
+  // before the byte under it is examined.
+  int64_t head = 0;
+  for (int64_t skipped = 0; skipped < skip_lines;) {
+    if (head >= size) return set_error(ENT_EINVAL, "%s has fewer than %lld lines to skip", path, (long long)skip_lines);
+    ++head;
+    if (head < size && f.data[head] == '\n') {
+      ++head;
+      ++skipped;
+    }
+  }
+  const char *d = f.data + head;
+  const int64_t len = size - head;
+  const char *end = d + len;
+  // Pass 1: token starts per chunk; pass 2: values at their global index.
+  const int nt = host_threads();
+  const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(int64_t(nt) * 8, len / 65536 + 1));
+  const int64_t csz = (len + nchunk - 1) / nchunk;
+  std::vector<int64_t> cnt(size_t(nchunk) + 1, 0);
+  parallel_for(nchunk, 1, [&](int64_t b, int64_t e) {
+    for (int64_t c = b; c < e; ++c) {
+      int64_t k = 0;
+      for (int64_t p = c * csz, pe = std::min(len, (c + 1) * csz); p < pe; ++p) k += token_start(d, p);
+      cnt[size_t(c) + 1] = k;
+    }
+  });
+  for (int64_t c = 0; c < nchunk; ++c) cnt[size_t(c) + 1] += cnt[size_t(c)];
+  const int64_t ntok = cnt[size_t(nchunk)];
+  if (ntok == 0) return set_error(ENT_EINVAL, "%s holds no tuples", path);
+  if (ntok & 1) return set_error(ENT_EINVAL, "%s holds an odd number of ids (%lld)", path, (long long)ntok);
+  std::vector<int64_t> tok(static_cast<size_t>(ntok));
+  std::vector<int64_t> lo(size_t(nchunk), INT64_MAX), hi(size_t(nchunk), INT64_MIN);
+  parallel_for(nchunk, 1, [&](int64_t b, int64_t e) {
+    for (int64_t c = b; c < e; ++c) {
+      int64_t k = cnt[size_t(c)], l = INT64_MAX, h = INT64_MIN;
+      for (int64_t p = c * csz, pe = std::min(len, (c + 1) * csz); p < pe; ++p)
+        if (token_start(d, p)) {
+          const int64_t x = text_atol(d + p, end);
+          tok[size_t(k++)] = x;
+          l = std::min(l, x);
+          h = std::max(h, x);
+        }
+      lo[size_t(c)] = l;
+      hi[size_t(c)] = h;
+    }
+  });
+  const int64_t vmin = *std::min_element(lo.begin(), lo.end());
+  const int64_t vmax = *std::max_element(hi.begin(), hi.end());
+  if (vmax - vmin >= int64_t(INT32_MAX))
+    return set_error(ENT_EINVAL, "ids span [%lld, %lld]: 2^31 or more vertices", (long long)vmin, (long long)vmax);
+  const int64_t ne = ntok / 2;
+  out->src = alloc_array<int32_t>(ne);
+  out->dst = alloc_array<int32_t>(ne);
+  if (!out->src || !out->dst) {
+    ent_tuples_free(out);
+    return set_error(ENT_ENOMEM, "out of host memory (%lld tuples)", (long long)ne);
+  }
+  parallel_for(ne, 1 << 16, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      out->src[i] = int32_t(tok[size_t(2 * i)] - vmin);
+      out->dst[i] = int32_t(tok[size_t(2 * i + 1)] - vmin);
+    }
+  });
+  out->n = vmax - vmin + 1;
+  out->ne = ne;
+  out->vmin = vmin;
+  return ENT_OK;
+}
+
+int ent_csr_write_ref_files(const ent_csr *g, const int32_t *weight, const char *prefix) {
+  clear_error();
+  if (!g || !g->beg_pos || (g->m > 0 && !g->adj) || !prefix) return set_error(ENT_EINVAL, "NULL argument");
+  const int64_t n = g->n, m = g->m;
+  // Each file is written from int64 values produced block by block.
+  auto write = [&](const char *suffix, int64_t count, auto value) -> int {
+    const std::string name = std::string(prefix) + suffix;
+    FILE *fp = std::fopen(name.c_str(), "wb");
+    if (!fp) return set_error(ENT_EIO, "cannot create %s: %s", name.c_str(), std::strerror(errno));
+    const int64_t blk = int64_t(1) << 20;
+    std::vector<int64_t> buf(size_t(std::min(blk, std::max<int64_t>(count, 1))));
+    for (int64_t done = 0; done < count;) {
+      const int64_t k = std::min(blk, count - done);
+      for (int64_t i = 0; i < k; ++i) buf[size_t(i)] = value(done + i);
+      if (std::fwrite(buf.data(), 8, size_t(k), fp) != size_t(k)) {
+        std::fclose(fp);
+        return set_error(ENT_EIO, "%s: short write", name.c_str());
+      }
+      done += k;
+    }
+    if (std::fclose(fp) != 0) return set_error(ENT_EIO, "%s: %s", name.c_str(), std::strerror(errno));
+    return ENT_OK;
+  };
+  int rc;
+  if ((rc = write("_beg_pos.bin", n + 1, [&](int64_t i) { return g->beg_pos[i]; }))) return rc;
+  if ((rc = write("_csr.bin", m, [&](int64_t i) { return int64_t(g->adj[i]); }))) return rc;
+  if ((rc = write("_deg.bin", n, [&](int64_t i) { return g->beg_pos[i + 1] - g->beg_pos[i]; }))) return rc;
+  // Row of entry i: advance a cursor over the rows (entries are visited in order).
+  int64_t row = 0;
+  if ((rc = write("_head.bin", m, [&](int64_t i) {
+         while (g->beg_pos[row + 1] <= i) ++row;
+         return row;
+       })))
+    return rc;
+  if (weight && (rc = write("_weight.bin", m, [&](int64_t i) { return int64_t(weight[i]); }))) return rc;
   return ENT_OK;
 }
 

@@ -13,6 +13,8 @@
  */
 #include "ent_oracle.h"
 
+#include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -216,4 +218,139 @@ void ent_oracle_report(int64_t n, const int64_t *beg_pos, const int32_t *depth,
     }
   if (reached_edges) *reached_edges = e;
   if (reached_vertices) *reached_vertices = c;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Edge list to CSR (the reference's offline converter).                    */
+
+int64_t ent_oracle_tuples_parse(const char *path, int64_t skip_lines, int64_t *tokens, int64_t cap,
+                                int64_t *vmin, int64_t *vmax) {
+  /* tuple_text_to_bin.cpp:66-147. The whole file is read into memory. */
+  FILE *f = fopen(path, "rb");
+  if (!f) return -1;
+  if (fseek(f, 0, SEEK_END) != 0) {
+    fclose(f);
+    return -1;
+  }
+  const long size = ftell(f);
+  rewind(f);
+  /* Two NUL bytes past the end: the token scan stops there (the reference
+   * reads past the end of a file that does not end in a separator). */
+  char *buf = (char *)malloc((size_t)size + 2);
+  if (!buf) {
+    fclose(f);
+    return -1;
+  }
+  if (size > 0 && fread(buf, 1, (size_t)size, f) != (size_t)size) {
+    free(buf);
+    fclose(f);
+    return -1;
+  }
+  fclose(f);
+  buf[size] = buf[size + 1] = 0;
+  /* Line skipping as at :80-97: the byte at the current offset is never
+   * examined before the offset advances, so a newline at offset 0 does not
+   * end a line. */
+  long head = 0;
+  int64_t skipped = 0;
+  while (skipped < skip_lines) {
+    if (head >= size) {
+      free(buf);
+      return -1; /* :86-91: more lines to skip than the file has */
+    }
+    head++;
+    if (head < size && buf[head] == '\n') {
+      head++;
+      skipped++;
+    }
+  }
+  const char *ss = buf + head;
+  const long len = size - head;
+  long next = 0, curr = 0;
+  int64_t count = 0, lo = INT64_MAX, hi = 0;
+  /* :112-139: one atol per token; ' ', '\n', '\t' separate tokens. */
+  while (next < len) {
+    const int64_t a = (int64_t)atol(ss + curr);
+    if (a > hi) hi = a;
+    if (a < lo) lo = a;
+    while (next < len && ss[next] != ' ' && ss[next] != '\n' && ss[next] != '\t') next++;
+    while (next < len && (ss[next] == ' ' || ss[next] == '\n' || ss[next] == '\t')) next++;
+    curr = next;
+    if (tokens && count < cap) tokens[count] = a;
+    count++;
+  }
+  free(buf);
+  if (vmin) *vmin = lo;
+  if (vmax) *vmax = hi;
+  return count;
+}
+
+typedef struct {
+  int32_t v;
+  int64_t pos;
+  int32_t w;
+} row_entry;
+
+static int row_entry_cmp(const void *x, const void *y) {
+  const row_entry *a = (const row_entry *)x, *b = (const row_entry *)y;
+  if (a->v != b->v) return a->v < b->v ? -1 : 1;
+  return a->pos < b->pos ? -1 : (a->pos > b->pos ? 1 : 0);
+}
+
+int64_t ent_oracle_csr_build(int64_t n, int64_t ne, const int32_t *src, const int32_t *dst,
+                             const int32_t *weight, int32_t flags, int64_t *beg_pos, int32_t *adj,
+                             int32_t *weight_out) {
+  /* tuple_text_to_bin.cpp:190-248: degrees per tuple (both ends when
+   * reversed), row starts by prefix sum, then every tuple stored in file
+   * order at its row's cursor, (u, v) in row u before (v, u) in row v. A
+   * reversed self-loop is stored twice in its row (the reference writes
+   * both copies to one slot and leaves the next slot unwritten). */
+  const int rev = (flags & 1) != 0, no_loops = (flags & 2) != 0;
+  const int dedup = (flags & 8) != 0, sorted = dedup || (flags & 4) != 0;
+  int64_t *deg = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  if (!deg) return -1;
+  for (int64_t i = 0; i < ne; ++i) {
+    if (no_loops && src[i] == dst[i]) continue;
+    deg[src[i]]++;
+    if (rev) deg[dst[i]]++;
+  }
+  beg_pos[0] = 0;
+  for (int64_t v = 0; v < n; ++v) beg_pos[v + 1] = beg_pos[v] + deg[v];
+  const int64_t m0 = beg_pos[n];
+  row_entry *ent = (row_entry *)malloc(sizeof(row_entry) * (size_t)(m0 > 0 ? m0 : 1));
+  if (!ent) {
+    free(deg);
+    return -1;
+  }
+  for (int64_t v = 0; v < n; ++v) deg[v] = beg_pos[v];
+  int64_t pos = 0;
+  for (int64_t i = 0; i < ne; ++i) {
+    const int32_t u = src[i], v = dst[i];
+    if (no_loops && u == v) continue;
+    const int32_t w = weight ? weight[i] : 0;
+    row_entry a = {v, pos++, w};
+    ent[deg[u]++] = a;
+    if (rev) {
+      row_entry b = {u, pos++, w};
+      ent[deg[v]++] = b;
+    }
+  }
+  /* Options beyond the reference: rows sorted by neighbour (ties in file
+   * order) and, with dedup, the first of each run of equal neighbours kept. */
+  int64_t m = 0;
+  for (int64_t v = 0; v < n; ++v) {
+    const int64_t s = beg_pos[v], e = beg_pos[v + 1];
+    if (sorted && e - s > 1) qsort(ent + s, (size_t)(e - s), sizeof(row_entry), row_entry_cmp);
+    beg_pos[v] = m;
+    for (int64_t p = s; p < e; ++p) {
+      if (dedup && p > s && ent[p].v == ent[p - 1].v) continue;
+      adj[m] = ent[p].v;
+      if (weight_out) weight_out[m] = ent[p].w;
+      m++;
+    }
+  }
+  beg_pos[n] = m;
+  free(ent);
+  free(deg);
+  return m;
 }

@@ -27,6 +27,12 @@
  *   ent_oracle_validate_parents  parents checked against depth (Graph500
  *                             rules) and planted wrong parents rejected
  *   ent_oracle_bfs_do         equal to ent_oracle_bfs on every test graph
+ *   ent_oracle_tuples_parse /
+ *   ent_oracle_csr_build      the reference converter itself, compiled from
+ *                             its source into oracle/_ref (tests/test_convert.py)
+ *                             and its committed outputs (tests/golden/convert);
+ *                             sorted and deduplicated rows against scipy's
+ *                             COO to CSR conversion
  */
 #ifndef ENT_ORACLE_H
 #define ENT_ORACLE_H
@@ -86,6 +92,28 @@ int32_t ent_oracle_validate_parents(int64_t n, const int64_t *beg_pos,
  * reference's TEPS numerator). */
 void ent_oracle_report(int64_t n, const int64_t *beg_pos, const int32_t *depth,
                        int64_t *reached_edges, int64_t *reached_vertices);
+
+/* The reference converter's text pass (tuple_text_to_bin.cpp:66-147): skips
+ * skip_lines lines the way it does, then reads tokens separated by ' ',
+ * '\t' and '\n' with atol. Returns the token count and stores up to cap
+ * values in tokens (may be NULL) and the extremes in *vmin, *vmax; -1 when
+ * the file cannot be read or has fewer lines than skip_lines. */
+int64_t ent_oracle_tuples_parse(const char *path, int64_t skip_lines,
+                                int64_t *tokens, int64_t cap, int64_t *vmin,
+                                int64_t *vmax);
+
+/* CSR of ne tuples (src[i], dst[i]) over n vertices, the reference
+ * converter's construction (tuple_text_to_bin.cpp:190-248) plus options.
+ * flags: 1 also store each tuple reversed, right after it; 2 drop (u, u);
+ * 4 sort each row by neighbour (ties in tuple order); 8 drop repeated
+ * neighbours of a row, keeping the first (implies 4). weight (NULL or ne
+ * entries) follows its tuple into weight_out (NULL or m entries).
+ * beg_pos: n + 1 entries; adj: capacity ne * 2. Returns m, or -1 when out
+ * of memory. */
+int64_t ent_oracle_csr_build(int64_t n, int64_t ne, const int32_t *src,
+                             const int32_t *dst, const int32_t *weight,
+                             int32_t flags, int64_t *beg_pos, int32_t *adj,
+                             int32_t *weight_out);
 
 #ifdef __cplusplus
 }

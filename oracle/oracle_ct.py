@@ -32,6 +32,11 @@ class Oracle:
         L.ent_oracle_validate_parents.argtypes = [C.c_int64, i64p, i32p, C.c_int32, i32p, i32p]
         L.ent_oracle_report.restype = None
         L.ent_oracle_report.argtypes = [C.c_int64, i64p, i32p, i64p, i64p]
+        L.ent_oracle_tuples_parse.restype = C.c_int64
+        L.ent_oracle_tuples_parse.argtypes = [C.c_char_p, C.c_int64, i64p, C.c_int64, i64p, i64p]
+        L.ent_oracle_csr_build.restype = C.c_int64
+        L.ent_oracle_csr_build.argtypes = [C.c_int64, C.c_int64, i32p, i32p, i32p, C.c_int32, i64p, i32p,
+                                           i32p]
         self.L = L
 
     @staticmethod
@@ -93,3 +98,32 @@ class Oracle:
         self.L.ent_oracle_report(b.size - 1, b.ctypes.data_as(i64p), d.ctypes.data_as(i32p),
                                  C.byref(e), C.byref(v))
         return e.value, v.value
+
+    def tuples_parse(self, path, skip_lines=0):
+        """(tokens int64[], vmin, vmax) as the reference converter reads a
+        text tuple file; None when it cannot (too few lines to skip)."""
+        lo, hi = C.c_int64(), C.c_int64()
+        cnt = self.L.ent_oracle_tuples_parse(str(path).encode(), int(skip_lines), None, 0, C.byref(lo), C.byref(hi))
+        if cnt < 0:
+            return None
+        t = np.empty(max(cnt, 1), dtype=np.int64)
+        self.L.ent_oracle_tuples_parse(str(path).encode(), int(skip_lines), t.ctypes.data_as(i64p), cnt,
+                                       C.byref(lo), C.byref(hi))
+        return t[:cnt], lo.value, hi.value
+
+    def csr_build(self, n, src, dst, flags, weight=None):
+        """(beg_pos int64[n+1], adj int32[m], weight int32[m] or None)."""
+        s = np.ascontiguousarray(src, dtype=np.int32)
+        d = np.ascontiguousarray(dst, dtype=np.int32)
+        ne = s.size
+        b = np.empty(n + 1, dtype=np.int64)
+        a = np.empty(max(2 * ne, 1), dtype=np.int32)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.int32)
+        wo = None if weight is None else np.empty(max(2 * ne, 1), dtype=np.int32)
+        m = self.L.ent_oracle_csr_build(int(n), ne, s.ctypes.data_as(i32p) if ne else None,
+                                        d.ctypes.data_as(i32p) if ne else None,
+                                        w.ctypes.data_as(i32p) if w is not None and ne else None, int(flags),
+                                        b.ctypes.data_as(i64p), a.ctypes.data_as(i32p),
+                                        wo.ctypes.data_as(i32p) if wo is not None else None)
+        assert m >= 0
+        return b, a[:m], (None if wo is None else wo[:m])
