@@ -10,17 +10,18 @@
 //   a dropped self-loop gets row = n and sorts behind every row.
 //
 // Sorted rows without weights sort the keys alone; the other forms carry the
-// record index (u32) to fetch the column and weight afterwards. Duplicate
-// removal is rocPRIM's unique (by key) over the sorted keys. One assembly
-// kernel then writes beg_pos (first entry of each row, from the row
-// boundaries of the sorted keys), the adjacency and the weights.
+// record index (u32) to fetch the column and weight afterwards. Then, per
+// tile of 4096 sorted records: the count of kept records (real row, and with
+// dedup a key unlike its predecessor's), a scan of the tile counts, and one
+// assembly pass that writes each kept record at its rank: its column and
+// weight, and beg_pos at the row boundaries.
 //
 // HBM traffic per record: the radix sort's passes over 8-byte keys (and
 // 4-byte indices), one pass per 8 key bits; the record and assembly kernels
 // read the tuples once and write the CSR once.
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_select.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -31,6 +32,7 @@
 #include "ent_bfs.h"
 #include "ent_common.h"
 #include "ent_convert.h"
+#include "ent_device.h"
 #include "ent_internal.h"
 
 namespace ent {
@@ -79,39 +81,102 @@ struct Gap {
 constexpr int kGapCap = 1 << 16;
 constexpr long long kGapSerial = 256;  // longest run a thread fills itself
 
-// beg[v] = first entry whose row is >= v, for v in [0, n]; entries of rows
-// < n get their column (from the key, or from the record idx[i]) and weight.
-// Entry i fills beg for the rows (row(i - 1), row(i)]; runs of more than
-// kGapSerial empty rows are listed for k_fill_gaps instead.
-__global__ __launch_bounds__(kBuildBlock) void k_assemble(BuildShape s, const unsigned long long *keys,
-                                                          const unsigned *idx, long long count, const int *src,
-                                                          const int *dst, const int *w_in, long long *beg, int *adj,
-                                                          int *w_out, Gap *gaps, unsigned long long *ngaps) {
-  const unsigned long long colmask = (1ull << s.cb) - 1;
-  for (long long i = (long long)blockIdx.x * kBuildBlock + threadIdx.x; i <= count;
-       i += (long long)gridDim.x * kBuildBlock) {
-    const int shift = s.sorted ? s.cb : 0;
-    const long long rp = i == 0 ? -1 : (long long)min(keys[i - 1] >> shift, (unsigned long long)s.n);
-    const long long rc = i == count ? (long long)s.n : (long long)min(keys[i] >> shift, (unsigned long long)s.n);
-    bool serial = rc - rp <= kGapSerial;
-    if (!serial) {
-      const unsigned long long g = atomicAdd(ngaps, 1ull);
-      if (g < (unsigned long long)kGapCap) gaps[g] = Gap{rp + 1, rc + 1, i};
-      else serial = true;
+// Sets beg[v] = val for v in [v0, v1): by this thread when the run is
+// short, else listed for k_fill_gaps (or by this thread if the list is full).
+__device__ __forceinline__ void fill_rows(long long *beg, long long v0, long long v1, long long val, Gap *gaps,
+                                          unsigned long long *ngaps) {
+  if (v1 - v0 > kGapSerial) {
+    const unsigned long long g = atomicAdd(ngaps, 1ull);
+    if (g < (unsigned long long)kGapCap) {
+      gaps[g] = Gap{v0, v1, val};
+      return;
     }
-    if (serial)
-      for (long long v = rp + 1; v <= rc; ++v) beg[v] = i;
-    if (i < count && rc < s.n) {
+  }
+  for (long long v = v0; v < v1; ++v) beg[v] = val;
+}
+
+// Tiles of kTile sorted records, one block each. Record i is kept when its
+// row is a real row (< n) and, with dedup, its key differs from record
+// i - 1's (records of a dropped loop sort last, behind every kept one).
+constexpr int kTileItems = 16;
+constexpr long long kTile = (long long)kBuildBlock * kTileItems;
+
+__device__ __forceinline__ bool record_kept(const BuildShape &s, const unsigned long long *keys, long long i,
+                                            int shift) {
+  const unsigned long long k = keys[i];
+  if ((k >> shift) >= (unsigned long long)s.n) return false;
+  return !s.dedup || i == 0 || keys[i - 1] != k;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_tile_count(BuildShape s, const unsigned long long *keys,
+                                                            unsigned long long *tile_cnt) {
+  const int shift = s.sorted ? s.cb : 0;
+  const long long t0 = (long long)blockIdx.x * kTile;
+  int c = 0;
+#pragma unroll 4
+  for (int j = 0; j < kTileItems; ++j) {
+    const long long i = t0 + (long long)j * kBuildBlock + threadIdx.x;
+    if (i < s.records) c += record_kept(s, keys, i, shift) ? 1 : 0;
+  }
+  __shared__ int part[kBuildBlock / kWave];
+  const int w = wave_sum(c);
+  if (lane_id() == 0) part[threadIdx.x / kWave] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int k = 0; k < kBuildBlock / kWave; ++k) t += (unsigned long long)part[k];
+    tile_cnt[blockIdx.x] = t;
+  }
+}
+
+// Writes each kept record at its rank (tile_off[tile] + rank in the tile):
+// its column and weight, and beg for the rows from the previous record's row
+// (exclusive) to its own (inclusive) when its row starts there; the last kept
+// record also fills beg for the rows after it, up to n. tile_off has
+// ntiles + 1 entries (the last is m); beg is zero beforehand (for m = 0).
+__global__ __launch_bounds__(kBuildBlock) void k_tile_assemble(BuildShape s, const unsigned long long *keys,
+                                                               const unsigned *idx,
+                                                               const unsigned long long *tile_off, const int *src,
+                                                               const int *dst, const int *w_in, long long *beg,
+                                                               int *adj, int *w_out, Gap *gaps,
+                                                               unsigned long long *ngaps) {
+  const int shift = s.sorted ? s.cb : 0;
+  const unsigned long long colmask = (1ull << s.cb) - 1;
+  const long long t0 = (long long)blockIdx.x * kTile;
+  const long long m = (long long)tile_off[gridDim.x];
+  long long base = (long long)tile_off[blockIdx.x];
+  const int wid = threadIdx.x / kWave, lane = lane_id();
+  __shared__ int wcnt[kBuildBlock / kWave];
+  for (int j = 0; j < kTileItems; ++j) {
+    const long long i = t0 + (long long)j * kBuildBlock + threadIdx.x;
+    const bool kp = i < s.records && record_kept(s, keys, i, shift);
+    const unsigned long long bal = __ballot(kp);
+    if (lane == 0) wcnt[wid] = __popcll(bal);
+    __syncthreads();
+    long long pos = base + mask_rank(bal);
+    long long step = 0;
+    for (int k = 0; k < kBuildBlock / kWave; ++k) {
+      if (k < wid) pos += wcnt[k];
+      step += wcnt[k];
+    }
+    if (kp) {
+      const unsigned long long key = keys[i];
+      const long long row = (long long)(key >> shift);
+      const long long prev = i == 0 ? -1 : (long long)(keys[i - 1] >> shift);
+      if (row != prev) fill_rows(beg, prev + 1, row + 1, pos, gaps, ngaps);
+      if (pos == m - 1) fill_rows(beg, row + 1, (long long)s.n + 1, m, gaps, ngaps);
       if (idx) {
         long long t;
-        unsigned long long row, col;
-        record_entry(s, src, dst, (long long)idx[i], t, row, col);
-        adj[i] = int(col);
-        if (w_out) w_out[i] = w_in[t];
+        unsigned long long r, col;
+        record_entry(s, src, dst, (long long)idx[i], t, r, col);
+        adj[pos] = int(col);
+        if (w_out) w_out[pos] = w_in[t];
       } else {
-        adj[i] = int(keys[i] & colmask);
+        adj[pos] = int(key & colmask);
       }
     }
+    base += step;
+    __syncthreads();
   }
 }
 
@@ -189,22 +254,19 @@ int build_csr(int device, hipStream_t stream, long long n, long long ne, const i
   } evg{e0, e1};
   rocprim::double_buffer<unsigned long long> keys(k0.as<unsigned long long>(), k1.as<unsigned long long>());
   rocprim::double_buffer<unsigned> vals(i0.as<unsigned>(), i1.as<unsigned>());
-  // Scratch for the sort and the unique pass, sized once.
-  size_t tb_sort = 0, tb_uniq = 0;
+  // Scratch for the sort and the scan of tile counts, sized once.
+  const long long ntiles = std::max<long long>(1, (s.records + kTile - 1) / kTile);
+  DevBuf tiles;
+  if ((rc = tiles.alloc(8 * size_t(2 * ntiles + 1), "tile counts"))) return rc;
+  unsigned long long *tile_cnt = tiles.as<unsigned long long>(), *tile_off = tile_cnt + ntiles;
+  size_t tb_sort = 0, tb_scan = 0;
   if (keys_only)
     HIP_TRY(rocprim::radix_sort_keys(nullptr, tb_sort, keys, R, 0u, unsigned(kbits), stream));
   else
     HIP_TRY(rocprim::radix_sort_pairs(nullptr, tb_sort, keys, vals, R, 0u, unsigned(kbits), stream));
-  if (s.dedup) {
-    if (keys_only)
-      HIP_TRY(rocprim::unique(nullptr, tb_uniq, k0.as<unsigned long long>(), k1.as<unsigned long long>(),
-                              cnt.as<unsigned long long>(), R, rocprim::equal_to<unsigned long long>(), stream));
-    else
-      HIP_TRY(rocprim::unique_by_key(nullptr, tb_uniq, k0.as<unsigned long long>(), i0.as<unsigned>(),
-                                     k1.as<unsigned long long>(), i1.as<unsigned>(), cnt.as<unsigned long long>(),
-                                     R, rocprim::equal_to<unsigned long long>(), stream));
-  }
-  if ((rc = tmp.alloc(std::max(tb_sort, tb_uniq), "sort scratch"))) return rc;
+  HIP_TRY(rocprim::inclusive_scan(nullptr, tb_scan, tile_cnt, tile_off + 1, size_t(ntiles),
+                                  rocprim::plus<unsigned long long>(), stream));
+  if ((rc = tmp.alloc(std::max(tb_sort, tb_scan), "sort scratch"))) return rc;
   HIP_TRY(hipEventRecord(e0, stream));
   if (R) {
     hipLaunchKernelGGL(k_records, dim3(grid_for(s.records, cus)), dim3(kBuildBlock), 0, stream, s, src, dst,
@@ -216,29 +278,24 @@ int build_csr(int device, hipStream_t stream, long long n, long long ne, const i
     else
       HIP_TRY(rocprim::radix_sort_pairs(tmp.p, tb, keys, vals, R, 0u, unsigned(kbits), stream));
   }
-  long long count = s.records;
-  if (s.dedup && R) {
-    size_t tb = tb_uniq;
-    if (keys_only)
-      HIP_TRY(rocprim::unique(tmp.p, tb, keys.current(), keys.alternate(), cnt.as<unsigned long long>(), R,
-                              rocprim::equal_to<unsigned long long>(), stream));
-    else
-      HIP_TRY(rocprim::unique_by_key(tmp.p, tb, keys.current(), vals.current(), keys.alternate(),
-                                     vals.alternate(), cnt.as<unsigned long long>(), R,
-                                     rocprim::equal_to<unsigned long long>(), stream));
-    unsigned long long h = 0;
-    HIP_TRY(hipMemcpyAsync(&h, cnt.p, 8, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
-    count = (long long)h;
-    keys.swap();
-    if (!keys_only) vals.swap();
-  }
+  // Kept records per tile, their running sum, then every kept record at its
+  // rank. beg starts at zero: with no kept record nothing else writes it.
   unsigned long long *ngaps = cnt.as<unsigned long long>() + 1;
   HIP_TRY(hipMemsetAsync(ngaps, 0, 8, stream));
-  hipLaunchKernelGGL(k_assemble, dim3(grid_for(count + 1, cus)), dim3(kBuildBlock), 0, stream, s, keys.current(),
-                     keys_only ? nullptr : vals.current(), count, src, dst, w_in, beg, adj, w_out, gaps.as<Gap>(),
-                     ngaps);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemsetAsync(tiles.p, 0, 8 * size_t(2 * ntiles + 1), stream));
+  HIP_TRY(hipMemsetAsync(beg, 0, 8 * size_t(n + 1), stream));
+  if (R) {
+    hipLaunchKernelGGL(k_tile_count, dim3(unsigned(ntiles)), dim3(kBuildBlock), 0, stream, s, keys.current(),
+                       tile_cnt);
+    HIP_TRY(hipGetLastError());
+    size_t tb = tb_scan;
+    HIP_TRY(rocprim::inclusive_scan(tmp.p, tb, tile_cnt, tile_off + 1, size_t(ntiles),
+                                    rocprim::plus<unsigned long long>(), stream));
+    hipLaunchKernelGGL(k_tile_assemble, dim3(unsigned(ntiles)), dim3(kBuildBlock), 0, stream, s, keys.current(),
+                       keys_only ? nullptr : vals.current(), tile_off, src, dst, w_in, beg, adj, w_out,
+                       gaps.as<Gap>(), ngaps);
+    HIP_TRY(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_fill_gaps, dim3(cus * 4), dim3(kBuildBlock), 0, stream, gaps.as<Gap>(), ngaps, beg);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(e1, stream));
