@@ -162,7 +162,9 @@ __global__ __launch_bounds__(kBuildBlock) void k_tile_assemble(BuildShape s, con
     if (kp) {
       const unsigned long long key = keys[i];
       const long long row = (long long)(key >> shift);
-      const long long prev = i == 0 ? -1 : (long long)(keys[i - 1] >> shift);
+      // Clamped: ids outside [0, n) (unchecked on the device form) cannot
+      // send beg writes out of range.
+      const long long prev = i == 0 ? -1 : (long long)min(keys[i - 1] >> shift, (unsigned long long)s.n);
       if (row != prev) fill_rows(beg, prev + 1, row + 1, pos, gaps, ngaps);
       if (pos == m - 1) fill_rows(beg, row + 1, (long long)s.n + 1, m, gaps, ngaps);
       if (idx) {

@@ -82,7 +82,9 @@ int ent_csr_build(int32_t device, int64_t n, int64_t ne, const int32_t *src,
 
 /* The same on device-resident arrays of `device`: src_dev, dst_dev (ne
  * each), beg_pos_dev (n + 1), adj_dev (adj_capacity >= records). *m_out
- * receives m. Scratch memory is allocated and released inside. */
+ * receives m. Scratch memory is allocated and released inside. The ids are
+ * not checked here (they are for host arrays): ids outside [0, n) give a
+ * wrong CSR, though every write stays inside the given arrays. */
 int ent_csr_build_device(int32_t device, int64_t n, int64_t ne,
                          const int32_t *src_dev, const int32_t *dst_dev,
                          int32_t flags, int64_t *beg_pos_dev, int32_t *adj_dev,
