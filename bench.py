@@ -163,9 +163,11 @@ def bench_plain(args):
         gpu_depths[i] = d
     # Schedule-exact examined bytes: the counting kernel variant over the
     # same roots (outside the timed region; it is slower).
-    dg.configure(count_examined=1)
-    exam = [dg.bfs_device(int(r))["examined_bytes"] for r in roots]
-    dg.configure(count_examined=0)
+    exam = []
+    if not args.no_examined:
+        dg.configure(count_examined=1)
+        exam = [dg.bfs_device(int(r))["examined_bytes"] for r in roots]
+        dg.configure(count_examined=0)
     used = [(r, dt, st) for (r, dt, st) in per if st["levels"] - 1 > 2]
     gteps = [st["reached_edges"] / dt / 1e9 for (r, dt, st) in used]
     gteps_dev = [st["reached_edges"] / (st["device_ms"] * 1e-3) / 1e9 for (r, dt, st) in used]
@@ -206,8 +208,8 @@ def bench_plain(args):
                      "kernel": "whole search (init + level kernels) per BFS, HIP events on "
                                "the handle's stream",
                      "alg_bytes_per_bfs_mean": int(np.mean(alg_bytes)),
-                     "examined_bytes_per_bfs_mean": int(np.mean(exam)),
-                     "examined_over_alg": round(float(np.mean(exam)) / float(np.mean(alg_bytes)), 3)},
+                     "examined_bytes_per_bfs_mean": int(np.mean(exam)) if exam else None,
+                     "examined_over_alg": round(float(np.mean(exam)) / float(np.mean(alg_bytes)), 3) if exam else None},
         "cpu_baseline": cpu,
         "extra": {
             "gteps_hmean_device_time": round(hmean(gteps_dev), 3),
@@ -239,6 +241,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--check-roots", type=int, default=4)
+    ap.add_argument("--no-examined", action="store_true",
+                    help="skip the examined-work pass (counting kernel variant) after the timed searches")
     ap.add_argument("--serve-scale", type=int, default=26)
     ap.add_argument("--serve-budget-gb", type=int, default=160)
     ap.add_argument("--rates", default="100,200,500,1000,2000,5000")
@@ -265,7 +269,7 @@ def main():
     if args.traffic:
         from enterprise_amd import traffic
         if args.mode == "plain":
-            child = ["--no-cpu", "--steps", str(args.steps), "--warmup", "0", "--check-roots", "0",
+            child = ["--no-cpu", "--no-examined", "--steps", str(args.steps), "--warmup", "0", "--check-roots", "0",
                      "--scale", str(args.scale)]
         elif args.mode == "batch":
             child = ["--mode", "batch", "--steps", "1", "--warmup", "0"]
