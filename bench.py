@@ -163,6 +163,12 @@ def bench_plain(args):
         gpu_depths[i] = d
     # Schedule-exact examined bytes: the counting kernel variant over the
     # same roots (outside the timed region; it is slower).
+    # Every timed root once more, each result checked on the device for all
+    # vertices (ent_bfs_validate_device; outside the timed region).
+    violations = 0
+    for r in roots:
+        dg.bfs_device(int(r))
+        violations += sum(dg.validate_device(int(r)))
     exam = []
     if not args.no_examined:
         dg.configure(count_examined=1)
@@ -222,6 +228,7 @@ def bench_plain(args):
             "upload_s": round(upload_s, 3),
             "roots_used": len(used),
             "parity": parity,
+            "device_validation": {"roots": len(roots), "violations": int(violations)},
             "cpu_serial_1core": cpu_serial,
             "gteps_hmean_pcie_inclusive": round(hmean(pcie), 3) if pcie else None,
         },

@@ -123,6 +123,16 @@ class DeviceGraph:
                                    C.c_void_p(depth_dev) if depth_dev else None, C.byref(st)))
         return st.as_dict()
 
+    def validate_device(self, root, depth_dev=None):
+        """All-vertex BFS check of a device depth array (default: the
+        handle's): [root depth != 0, bad values, vertices with a bad edge,
+        reached vertices without a parent]; all zero iff it is the BFS from
+        root."""
+        out = np.zeros(4, dtype=np.int64)
+        ptr = depth_dev if depth_dev is not None else self.depth_device_ptr()
+        check(lib().ent_bfs_validate_device(self.handle, int(root), C.c_void_p(ptr), out.ctypes.data_as(i64p)))
+        return [int(x) for x in out]
+
     def level_trace(self):
         """Per-launch records of the last search run with count_examined=1:
         list of dicts (launch, level, phase name, frontier vertices and edges)."""

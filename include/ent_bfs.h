@@ -150,6 +150,20 @@ int ent_graph_free(ent_graph *g);
 int ent_bfs_parents_device(ent_graph *g, int32_t root, const int32_t *depth_dev,
                            int32_t *parent_dev);
 
+/* Every-vertex check of a depth array on the device (depth_dev: n entries),
+ * the invariants behind the reference's spot check (validate.h:31-103)
+ * applied to all vertices instead of random samples:
+ *   counts[0] = 1 if depth[root] != 0;
+ *   counts[1] = vertices whose depth is below -1 or at least n;
+ *   counts[2] = vertices with a neighbour whose depth differs from theirs by
+ *               more than one, or with exactly one of the two unreached;
+ *   counts[3] = reached vertices other than root with no neighbour one level
+ *               closer.
+ * All four are zero exactly when depth is the BFS from root (hop distances
+ * are unique). Returns ENT_OK whatever the counts. */
+int ent_bfs_validate_device(ent_graph *g, int32_t root, const int32_t *depth_dev,
+                            int64_t counts[4]);
+
 /* One search from root, then its parents; depth_host and parent_host (n
  * entries, either may be NULL) receive both arrays. */
 int ent_bfs_parents(ent_graph *g, int32_t root, int32_t *depth_host,

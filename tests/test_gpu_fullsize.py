@@ -33,6 +33,12 @@ def test_rmat22_roots_bit_exact(oracle):
             assert (st["reached_edges"], st["reached_vertices"]) == (edges, verts)
         d, p, _ = dg.bfs_parents(int(roots[5]))
         assert oracle.validate_parents(g.beg_pos, g.adj, int(roots[5]), d, p) == 0
+        # Every benchmark root: the device all-vertex check (zero counts
+        # exactly for the BFS from the root).
+        _say("[gpu] rmat22 validate 64 roots on the device")
+        for r in roots:
+            dg.bfs_device(int(r))
+            assert dg.validate_device(int(r)) == [0, 0, 0, 0], int(r)
 
 
 def test_rmat23_batch_512_rows(oracle):
@@ -55,6 +61,9 @@ def test_rmat23_batch_512_rows(oracle):
             for i in (0, 255, 256, 511):
                 want = oracle.bfs(g.beg_pos, g.adj, int(roots[i]))
                 assert int(h[i]) == int(row_hash_host(want)), i
+            _say("[gpu] rmat23 validate 512 rows on the device")
+            for i, r in enumerate(roots):
+                assert dg.validate_device(int(r), out + 4 * g.n * i) == [0, 0, 0, 0], i
         finally:
             dg.device_free(out)
 
@@ -68,3 +77,6 @@ def test_grid4096_root(oracle):
         want = oracle.bfs(g.beg_pos, g.adj, r)
         assert np.array_equal(got, want)
         assert st["levels"] == int(want.max()) + 1 and st["levels"] > 4000
+        for r2 in g.pick_roots(16, 1003):
+            dg.bfs_device(int(r2))
+            assert dg.validate_device(int(r2)) == [0, 0, 0, 0], int(r2)
