@@ -1770,6 +1770,12 @@ int ent_bfs_gpu_coalescing_mem(const int32_t *src_list, int32_t nsrc, const int6
     const double sec = std::chrono::duration<double>(t1 - t0).count();
     if (st.levels - 1 > 2) {
       ++counted;
+      // The reference's check samples vertices and prints CORRECT whatever it
+      // finds (bfs_gpu_opt.cuh:418-425); this one checks every vertex on the
+      // device and prints what it finds.
+      int64_t bad[4] = {0, 0, 0, 0};
+      if ((rc = ent_bfs_validate_device(g, src_list[i], g->depth, bad))) break;
+      std::printf("\nBFS result validation: %s\n", bad[0] + bad[1] + bad[2] + bad[3] ? "WRONG" : "CORRECT");
       const double teps = double(st.reached_edges) / (1e9 * sec);
       avg = (teps + avg * (counted - 1)) / counted;
       std::printf("Traversed vertices: %lld\t\t\tTraversed edges: %lld\nTraversed time(s) :%g\t\t"

@@ -34,6 +34,10 @@ def _check_report(out, g):
     assert all(deg[int(r)] > 0 for _, r in tests)          # non-isolated roots (main.cu:55-78)
     avg = re.findall(r"Final Average TEPS \(Billion\): ([0-9.eE+-]+)", out)
     assert len(avg) == 1 and float(avg[0]) > 0
+    # Every counted root (more than 2 levels) is validated on the device.
+    counted = len(re.findall(r"Current TEPS", out))
+    assert counted > 0 and out.count("BFS result validation: CORRECT") == counted
+    assert "WRONG" not in out
 
 
 def test_cli_reads_reference_files(tmp_path):
