@@ -77,6 +77,22 @@ def cpu_baseline_serial(g, roots, budget_s, gpu_depths=None):
                       f"{os.cpu_count()} host cores visible"}, checked, mismatches
 
 
+def grid_cpu_check(g, checks):
+    """Grid mode: the serial oracle's time per BFS (the CPU baseline) and
+    parity of the GPU depths it was handed, on the same roots."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ct
+    oracle = oracle_ct.Oracle()
+    cpu_ms, mism = [], 0
+    for r, got in checks:
+        t0 = time.perf_counter()
+        want = oracle.bfs(g.beg_pos, g.adj, int(r))
+        cpu_ms.append((time.perf_counter() - t0) * 1e3)
+        mism += int(not np.array_equal(got, want))
+    return {"cpu_serial_ms_per_bfs": round(float(np.mean(cpu_ms)), 1), "parity_roots": len(checks),
+            "parity_mismatches": mism}
+
+
 def cpu_baseline_openmp(g, roots, budget_s, threads):
     """Direction-optimizing OpenMP BFS of the oracle on `threads` host cores
     over a bounded sample of the same roots."""
@@ -292,6 +308,9 @@ def main():
     else:
         from enterprise_amd import benchmarks
         out = benchmarks.run(args)
+        chk = out.pop("_cpu_check", None)
+        if chk is not None:
+            out.update(grid_cpu_check(chk["graph"], chk["checks"]))
     print(json.dumps(out), flush=True)
 
 
