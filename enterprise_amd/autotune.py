@@ -37,6 +37,7 @@ SPACE = {
     "alpha": [4, 8, 15, 30, 60],
     "beta": [6, 12, 18, 36, 72],
     "queue_small": [-1, 1024, 4096, 16384, 65536],
+    "queue_max_degree": [0, 64, 256, 4096, 32768],
     "hub_cache": [0, 8, 10, 12],
     "hub_td": [0, 1],
 }
@@ -199,6 +200,8 @@ def main():
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--budget", type=float, default=150.0)
     ap.add_argument("--out", default=None, help="output directory for the JSON")
+    ap.add_argument("--from-tuned", action="store_true",
+                    help="start the descent from the saved configuration for this graph, if any")
     ap.add_argument("--batch", type=int, default=0,
                     help="tune the multi-source search with this many roots (half of them from the "
                          "top-1%% degree vertices) instead of the single-source search")
@@ -216,6 +219,9 @@ def main():
                 "score": "aggregate GTEPS of one multi-source search"}
     else:
         roots = g.pick_roots(args.roots, 777)
+        start = load(f"rmat{args.scale}") if args.from_tuned else None
+        if start:
+            dg.configure(**{k: v for k, v in start.items() if k not in ("reserved", "launch")})
         best, score, hist = tune(dg, roots, roots[:3], rounds=args.rounds, log=log, budget_s=args.budget)
         name = f"rmat{args.scale}"
         meta = {"graph": f"rmat{args.scale} ef16 seed {args.scale}", "roots": args.roots,
