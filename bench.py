@@ -285,8 +285,8 @@ def main():
                     help="config override key=value (ent_bfs_config field), repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(batched and serving work shard by replicating the graph, see DESIGN.md)",
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(batched and serving work would shard by replicating the graph across GPUs)",
               file=sys.stderr)
         sys.exit(2)
     if args.traffic:
