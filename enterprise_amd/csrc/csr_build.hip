@@ -1,6 +1,6 @@
 // Edge list to CSR on the device, behind include/ent_convert.h. The
 // reference builds its CSR on the host, tuple by tuple, in file order
-// (tuple_text_to_binary_csr/tuple_text_to_bin.cpp:190-248); this build
+// (tuple_text_to_binary_csr/tuple_text_to_bin.cpp:190-248); this file
 // gives every stored entry one 64-bit record and radix-sorts the records:
 //
 //   record r (tuple i = r / 2 with ENT_CSR_REVERSE, else i = r; the odd
