@@ -93,6 +93,22 @@ def grid_cpu_check(g, checks):
             "parity_mismatches": mism}
 
 
+def csr_cpu_baseline(n, src, dst, sample=64 << 20):
+    """CSR mode: the oracle's single-core construction (same flags: both
+    directions, no loops, sorted deduplicated rows) on the first `sample`
+    tuples, in million tuples per second."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ct
+    oracle = oracle_ct.Oracle()
+    k = min(sample, int(src.size))
+    t0 = time.perf_counter()
+    oracle.csr_build(n, src[:k], dst[:k], 1 | 2 | 8)
+    dt = time.perf_counter() - t0
+    return {"value": round(k / dt / 1e6, 2), "unit": "Mtuples/s", "cores": 1, "kind": "port",
+            "sample": f"oracle construction (restated converter, sorted and deduplicated) on the first {k} "
+                      f"of {src.size} tuples", "ms": round(dt * 1e3, 1)}
+
+
 def cpu_baseline_openmp(g, roots, budget_s, threads):
     """Direction-optimizing OpenMP BFS of the oracle on `threads` host cores
     over a bounded sample of the same roots."""
@@ -311,6 +327,9 @@ def main():
         chk = out.pop("_cpu_check", None)
         if chk is not None:
             out.update(grid_cpu_check(chk["graph"], chk["checks"]))
+        csr = out.pop("_cpu_csr", None)
+        if csr is not None:
+            out["cpu_baseline"] = csr_cpu_baseline(csr["n"], csr["src"], csr["dst"])
     print(json.dumps(out), flush=True)
 
 
