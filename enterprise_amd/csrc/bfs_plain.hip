@@ -100,6 +100,9 @@ struct State {
   int pad2;
   unsigned int qn[kStripes];  // entries per queue stripe
 };
+// Leading words of State the host reads after a search (everything before
+// qn), mirrored by block 0 into host memory after every launch.
+constexpr int kHostStateWords = int(offsetof(State, qn) / 4);
 
 // Per-launch record of the counting variant (ent_level_trace).
 struct LaunchRec {
@@ -152,6 +155,7 @@ struct Args {
   unsigned char *mark;
   int *qbuf[2];  // striped frontier queues: stripe s at qbuf[p] + s * qcap_s
   Control *ctl;
+  State *hst;  // host-coherent mirror of ctl->st (leading kHostStateWords words)
   long long m;
   long long n_active;
   int n;
@@ -859,6 +863,9 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
       const unsigned *s32 = reinterpret_cast<const unsigned *>(&sm.S);
       unsigned *d32 = reinterpret_cast<unsigned *>(&c->st[i & 1]);
       for (int w = lane; w < int(sizeof(State) / 4); w += kWave) st_agent(d32 + w, s32[w]);
+      // The host polls this copy after a stream synchronisation, with no
+      // device-to-host copy of its own.
+      if (lane < kHostStateWords) reinterpret_cast<unsigned *>(&a.hst[i & 1])[lane] = s32[lane];
     }
     if (blockIdx.x == 0) {
       unsigned long long *z = &c->cnt[(i + 1) % 3].v[lane * kStripeU64];
@@ -1166,6 +1173,7 @@ Args make_args(ent_graph *g, int *depth) {
   else a.q_small = int(std::min<int64_t>(std::max<int64_t>(4096, g->n / 256), int64_t(kStripes) * g->qcap_s / 2));
   a.q_maxdeg = g->cfg.queue_max_degree > 0 ? g->cfg.queue_max_degree : g->cfg.t_block;
   a.ctl = g->ctl;
+  a.hst = g->h_st_dev ? g->h_st_dev->st : nullptr;
   a.m = g->m;
   a.n_active = g->n_active;
   a.n = int(g->n);
@@ -1266,9 +1274,8 @@ hipError_t queue_phases(ent_graph *g, const KernelSet &ks, Args *a, long long i0
 
 // Copies both state slots to the host and returns the latest (launch last).
 int poll_state(ent_graph *g, long long last, State *out) {
-  HIP_TRY(hipMemcpyAsync(g->h_ctl, g->ctl, sizeof(State) * 2, hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
-  *out = g->h_ctl->st[last & 1];
+  std::memcpy(out, &g->h_ctl->st[last & 1], sizeof(unsigned) * kHostStateWords);
   return ENT_OK;
 }
 
@@ -1301,7 +1308,6 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
       if (cfg.grid_blocks > 0) g->last_grid = std::min(g->last_grid, cfg.grid_blocks);
       HIP_TRY(hipLaunchCooperativeKernel(ks.persist, dim3(g->last_grid), dim3(bs), args,
                                          unsigned(hub_lds_bytes(cfg)), g->stream));
-      HIP_TRY(hipMemcpyAsync(g->h_ctl, g->ctl, sizeof(State) * 2, hipMemcpyDeviceToHost, g->stream));
       HIP_TRY(hipStreamSynchronize(g->stream));
       const State &s0 = g->h_ctl->st[0], &s1 = g->h_ctl->st[1];
       *fin = s0.phases >= s1.phases ? s0 : s1;
@@ -1386,7 +1392,10 @@ int alloc_workspace(ent_graph *g) {
       (rc = dev_alloc(g, reinterpret_cast<void **>(&g->ctl), sizeof(Control))) ||
       (rc = dev_alloc(g, &g->aux, 256)))
     return rc;
-  HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_ctl), sizeof(Control), hipHostMallocDefault));
+  // Host-coherent: the level kernels write the state mirror here directly.
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_ctl), sizeof(Control), hipHostMallocCoherent));
+  std::memset(g->h_ctl, 0, sizeof(Control));
+  HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void **>(&g->h_st_dev), g->h_ctl, 0));
   HIP_TRY(hipMemset(g->ctl, 0, sizeof(Control)));
   HIP_TRY(hipDeviceSynchronize());
   return build_lists(g);

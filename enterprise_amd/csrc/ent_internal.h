@@ -75,7 +75,8 @@ struct ent_graph {
   int *qbuf[2] = {nullptr, nullptr};  // striped frontier queues (kQStripes x qcap_s)
   int qcap_s = 0;
   ent::Control *ctl = nullptr;
-  ent::Control *h_ctl = nullptr;
+  ent::Control *h_ctl = nullptr;     // host-coherent; the kernels mirror their state into h_ctl->st
+  ent::Control *h_st_dev = nullptr;  // the same memory as a device pointer
   ent_bfs_config cfg{};
   int cus = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
