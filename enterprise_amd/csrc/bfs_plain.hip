@@ -1308,6 +1308,10 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
       if (cfg.grid_blocks > 0) g->last_grid = std::min(g->last_grid, cfg.grid_blocks);
       HIP_TRY(hipLaunchCooperativeKernel(ks.persist, dim3(g->last_grid), dim3(bs), args,
                                          unsigned(hub_lds_bytes(cfg)), g->stream));
+      HIP_TRY(hipEventRecord(g->ev1, g->stream));
+      // One launch per search: the device copy of both states (it also holds
+      // the barrier-timeout flag, which any block may set).
+      HIP_TRY(hipMemcpyAsync(g->h_ctl, g->ctl, sizeof(State) * 2, hipMemcpyDeviceToHost, g->stream));
       HIP_TRY(hipStreamSynchronize(g->stream));
       const State &s0 = g->h_ctl->st[0], &s1 = g->h_ctl->st[1];
       *fin = s0.phases >= s1.phases ? s0 : s1;
@@ -1322,6 +1326,7 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
       int k = cfg.launch == ENT_LAUNCH_LEVELS ? 1 : std::max(2, g->expect_launches);
       for (;;) {
         HIP_TRY(queue_phases(g, ks, &a, i, k));
+        HIP_TRY(hipEventRecord(g->ev1, g->stream));  // end of the search once it is done
         i += k;
         rc = poll_state(g, i - 1, fin);
         if (rc) return rc;
@@ -1360,6 +1365,7 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
       g->last_grid = grid;
       for (long long i = 0;; i += k) {
         HIP_TRY(hipGraphLaunch(g->gexec, g->stream));
+        HIP_TRY(hipEventRecord(g->ev1, g->stream));
         rc = poll_state(g, k - 1, fin);
         if (rc) return rc;
         if (fin->done) break;
@@ -1663,8 +1669,7 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
     ignore(hipStreamSynchronize(g->stream));
     return rc;
   }
-  HIP_TRY(hipEventRecord(g->ev1, g->stream));
-  HIP_TRY(hipEventSynchronize(g->ev1));
+  // ev1 was recorded behind the last launch and the stream has drained.
   const auto t1 = std::chrono::steady_clock::now();
   if (c.error)
     return set_error(ENT_EINTERNAL, "device reported error flags 0x%x (2 level guard, 4 barrier timeout)", c.error);
