@@ -123,6 +123,12 @@ class DeviceGraph:
                                    C.c_void_p(depth_dev) if depth_dev else None, C.byref(st)))
         return st.as_dict()
 
+    def last_device_ms(self):
+        """HIP-event device time of the last search (waits for its launches)."""
+        ms = C.c_double()
+        check(lib().ent_last_device_ms(self.handle, C.byref(ms)))
+        return ms.value
+
     def validate_device(self, root, depth_dev=None):
         """All-vertex BFS check of a device depth array (default: the
         handle's): [root depth != 0, bad values, vertices with a bad edge,

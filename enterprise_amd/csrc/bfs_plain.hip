@@ -94,15 +94,25 @@ struct State {
   unsigned long long total_mf;
   unsigned long long prev_nf;
   unsigned long long bu_mask;
+  unsigned long long t_begin;  // constant 100 MHz clock at k_init
+  unsigned long long t_end;    // ... at the launch that found the search finished
   int bm_valid;  // frontier of `level` is in front[level & 1]
   int q_valid;   // frontier of `level` is in the striped queue qbuf[level & 1]
   int pending;   // phase to run after PH_REBUILD
-  int pad2;
+  int search;    // the host's search number, carried from launch to launch
   unsigned int qn[kStripes];  // entries per queue stripe
 };
 // Leading words of State the host reads after a search (everything before
 // qn), mirrored by block 0 into host memory after every launch.
 constexpr int kHostStateWords = int(offsetof(State, qn) / 4);
+// The mirror holds each word in the low half of a 64-bit word whose high half
+// is the tag of the launch that wrote it: launch + search * kTagMul (mod
+// 2^32). The lanes' stores reach host memory in no fixed order, so the host
+// takes a slot only when all its tags agree. Tags of two launches of one
+// search, or of consecutive searches, differ while launch indices stay below
+// kSpinMaxLaunch (the host polls after a stream synchronisation beyond it).
+constexpr unsigned kTagMul = 0x9E3779B1u;
+constexpr long long kSpinMaxLaunch = 1ll << 24;
 
 // Per-launch record of the counting variant (ent_level_trace).
 struct LaunchRec {
@@ -155,7 +165,7 @@ struct Args {
   unsigned char *mark;
   int *qbuf[2];  // striped frontier queues: stripe s at qbuf[p] + s * qcap_s
   Control *ctl;
-  State *hst;  // host-coherent mirror of ctl->st (leading kHostStateWords words)
+  unsigned long long *hmir;  // host-coherent tagged mirror: 2 slots x kHostStateWords
   long long m;
   long long n_active;
   int n;
@@ -744,11 +754,13 @@ __device__ __forceinline__ void next_state(const Args &a, State &s, const unsign
   if (nf == 0) {
     s.levels = L + 1;
     s.done = 1;
+    s.t_end = rt_clock();
     return;
   }
   if (L + 1 >= a.n) {
     s.error |= ERR_LEVELS;
     s.done = 1;
+    s.t_end = rt_clock();
     return;
   }
   const bool cur_bu = pphase == PH_BU;
@@ -863,9 +875,11 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
       const unsigned *s32 = reinterpret_cast<const unsigned *>(&sm.S);
       unsigned *d32 = reinterpret_cast<unsigned *>(&c->st[i & 1]);
       for (int w = lane; w < int(sizeof(State) / 4); w += kWave) st_agent(d32 + w, s32[w]);
-      // The host polls this copy after a stream synchronisation, with no
-      // device-to-host copy of its own.
-      if (lane < kHostStateWords) reinterpret_cast<unsigned *>(&a.hst[i & 1])[lane] = s32[lane];
+      // Tagged host copy: the host reads it without a device-to-host copy.
+      if (lane < kHostStateWords) {
+        const unsigned tag = unsigned(i) + unsigned(sm.S.search) * kTagMul;
+        a.hmir[(i & 1) * kHostStateWords + lane] = (static_cast<unsigned long long>(tag) << 32) | s32[lane];
+      }
     }
     if (blockIdx.x == 0) {
       unsigned long long *z = &c->cnt[(i + 1) % 3].v[lane * kStripeU64];
@@ -879,7 +893,7 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
 // front[0] = {root}, marks cleared. The state before launch 0 is a finished
 // pseudo-level -1 whose counts (cnt[2]) describe frontier 0 = {root}.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu) {
+__global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu, int search) {
   const long long gt = (long long)blockIdx.x * BS + threadIdx.x;
   const long long nt = (long long)gridDim.x * BS;
   for (long long v = gt; v < a.n; v += nt) {
@@ -909,6 +923,8 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu) {
     State S{};
     S.level = -1;
     S.phase = first_bu ? PH_BU : PH_INSPECT;
+    S.search = search;
+    S.t_begin = rt_clock();
     c->st[1] = S;
     c->st[0] = S;
     c->sync.arrive_all = 0;
@@ -1009,6 +1025,7 @@ namespace {
 void release(ent_graph *g) {
   if (!g) return;
   ignore(hipSetDevice(g->device));
+  if (g->stream) ignore(hipStreamSynchronize(g->stream));  // launches queued behind a finished search
   batch_release(g);
   if (g->gexec) ignore(hipGraphExecDestroy(g->gexec));
   if (g->graph) ignore(hipGraphDestroy(g->graph));
@@ -1029,6 +1046,7 @@ void release(ent_graph *g) {
   ignore(hipFree(g->qbuf[1]));
   ignore(hipFree(g->ctl));
   if (g->h_ctl) ignore(hipHostFree(g->h_ctl));
+  if (g->h_mir) ignore(hipHostFree(g->h_mir));
   for (int b = 0; b < 2; ++b) {
     if (g->stage[b]) ignore(hipHostFree(g->stage[b]));
     if (g->stage_ev[b]) ignore(hipEventDestroy(g->stage_ev[b]));
@@ -1173,7 +1191,7 @@ Args make_args(ent_graph *g, int *depth) {
   else a.q_small = int(std::min<int64_t>(std::max<int64_t>(4096, g->n / 256), int64_t(kStripes) * g->qcap_s / 2));
   a.q_maxdeg = g->cfg.queue_max_degree > 0 ? g->cfg.queue_max_degree : g->cfg.t_block;
   a.ctl = g->ctl;
-  a.hst = g->h_st_dev ? g->h_st_dev->st : nullptr;
+  a.hmir = g->h_mir_dev;
   a.m = g->m;
   a.n_active = g->n_active;
   a.n = int(g->n);
@@ -1272,11 +1290,60 @@ hipError_t queue_phases(ent_graph *g, const KernelSet &ks, Args *a, long long i0
   return hipSuccess;
 }
 
-// Copies both state slots to the host and returns the latest (launch last).
+// Decodes mirror slot `slot` into the leading words of *out and its tag into
+// *tag; false if the words carry different tags (a slot being rewritten).
+bool read_slot(const ent_graph *g, int slot, State *out, unsigned *tag) {
+  const volatile unsigned long long *m = g->h_mir + slot * kHostStateWords;
+  unsigned w[kHostStateWords];
+  const unsigned long long x0 = m[0];
+  const unsigned t0 = unsigned(x0 >> 32);
+  bool same = true;
+  w[0] = unsigned(x0);
+  for (int k = 1; k < kHostStateWords; ++k) {
+    const unsigned long long x = m[k];
+    same = same && unsigned(x >> 32) == t0;
+    w[k] = unsigned(x);
+  }
+  std::memcpy(out, w, sizeof w);
+  *tag = t0;
+  return same;
+}
+
+// State after launch `last` (the last one queued), read once the stream has
+// drained.
 int poll_state(ent_graph *g, long long last, State *out) {
   HIP_TRY(hipStreamSynchronize(g->stream));
-  std::memcpy(out, &g->h_ctl->st[last & 1], sizeof(unsigned) * kHostStateWords);
+  unsigned tag;
+  read_slot(g, int(last & 1), out, &tag);
   return ENT_OK;
+}
+
+// Waits, spinning on the host-coherent mirror, until this search (number
+// `search`) is finished or launch `last` (the last one queued) has published
+// its state, and returns that state. The launch that finds the search
+// finished starts only after every earlier launch has completed, so the
+// results are final; launches queued behind it (they exit at once) may still
+// be running on the stream. The stream is queried now and then, so an error
+// or a drained stream ends the wait too.
+int wait_state(ent_graph *g, long long last, unsigned search, State *out) {
+  if (last >= kSpinMaxLaunch) return poll_state(g, last, out);
+  const unsigned want = unsigned(last) + search * kTagMul;
+  State s;
+  unsigned tag;
+  for (unsigned long spins = 0;; ++spins) {
+    for (int sl = 0; sl < 2; ++sl) {
+      if (!read_slot(g, sl, &s, &tag) || unsigned(s.search) != search) continue;
+      if (s.done || tag == want) {
+        std::memcpy(out, &s, sizeof(unsigned) * kHostStateWords);
+        return ENT_OK;
+      }
+    }
+    if ((spins & 255) == 255) {
+      const hipError_t e = hipStreamQuery(g->stream);
+      if (e == hipSuccess) return poll_state(g, last, out);  // drained: the mirror is complete
+      if (e != hipErrorNotReady) return set_error(ENT_EDEVICE, "search failed: %s", hipGetErrorString(e));
+    }
+  }
 }
 
 // Runs one search; on success *fin holds the final state.
@@ -1293,8 +1360,10 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
   Args a = make_args(g, depth_dev);
   const int bs = cfg.block_size;
   int first_bu = cfg.direction == ENT_DIR_BOTTOMUP ? 1 : 0;
+  if (++g->search_seq == 0) g->search_seq = 1;  // 0 marks the zeroed mirror
+  int search = int(g->search_seq);
   {
-    void *args[] = {&a, &root, &first_bu};
+    void *args[] = {&a, &root, &first_bu, &search};
     const int grid = int(std::min<int64_t>(int64_t(g->cus) * 8, (g->n + bs - 1) / bs + 1));
     HIP_TRY(launch(ks.init, std::max(1, grid), bs, args, g->stream));
   }
@@ -1328,7 +1397,7 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
         HIP_TRY(queue_phases(g, ks, &a, i, k));
         HIP_TRY(hipEventRecord(g->ev1, g->stream));  // end of the search once it is done
         i += k;
-        rc = poll_state(g, i - 1, fin);
+        rc = wait_state(g, i - 1, g->search_seq, fin);
         if (rc) return rc;
         if (fin->done) break;
         if (i > max_launches) return set_error(ENT_EINTERNAL, "search did not finish in %lld launches", i);
@@ -1398,10 +1467,14 @@ int alloc_workspace(ent_graph *g) {
       (rc = dev_alloc(g, reinterpret_cast<void **>(&g->ctl), sizeof(Control))) ||
       (rc = dev_alloc(g, &g->aux, 256)))
     return rc;
-  // Host-coherent: the level kernels write the state mirror here directly.
-  HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_ctl), sizeof(Control), hipHostMallocCoherent));
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_ctl), sizeof(Control), hipHostMallocDefault));
   std::memset(g->h_ctl, 0, sizeof(Control));
-  HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void **>(&g->h_st_dev), g->h_ctl, 0));
+  // Host-coherent: the level kernels write the tagged state mirror here.
+  // Zeroed words carry search 0, which no search uses.
+  const size_t mir_bytes = sizeof(unsigned long long) * 2 * kHostStateWords;
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_mir), mir_bytes, hipHostMallocCoherent));
+  std::memset(g->h_mir, 0, mir_bytes);
+  HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void **>(&g->h_mir_dev), g->h_mir, 0));
   HIP_TRY(hipMemset(g->ctl, 0, sizeof(Control)));
   HIP_TRY(hipDeviceSynchronize());
   return build_lists(g);
@@ -1596,6 +1669,7 @@ int ent_level_trace(ent_graph *g, ent_level_record *out, int32_t max, int32_t *c
   if (g->trace_launches <= 0) return ENT_OK;
   HIP_TRY(hipSetDevice(g->device));
   const int k = std::min(g->trace_launches, kTraceMax);
+  HIP_TRY(hipStreamSynchronize(g->stream));
   std::vector<LaunchRec> rec(size_t(std::max(k, 1)));
   HIP_TRY(hipMemcpy(rec.data(), g->ctl->trace, sizeof(LaunchRec) * size_t(k), hipMemcpyDeviceToHost));
   const int nout = std::min(k, int(max));
@@ -1669,7 +1743,8 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
     ignore(hipStreamSynchronize(g->stream));
     return rc;
   }
-  // ev1 was recorded behind the last launch and the stream has drained.
+  // The search is finished; launches queued behind the one that found it
+  // finished may still be on the stream (they exit at once).
   const auto t1 = std::chrono::steady_clock::now();
   if (c.error)
     return set_error(ENT_EINTERNAL, "device reported error flags 0x%x (2 level guard, 4 barrier timeout)", c.error);
@@ -1685,8 +1760,9 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
     st->reached_vertices = int64_t(c.total_nf);
     st->reached_edges = int64_t(c.total_mf);
     st->alg_bytes = 8 * st->reached_edges + 12 * st->reached_vertices + 4 * g->n;
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, g->ev0, g->ev1) == hipSuccess) st->device_ms = ms;
+    // Device clock (100 MHz) from k_init to the launch that found the search
+    // finished; the HIP-event time of the same search is ent_last_device_ms.
+    if (c.t_end > c.t_begin) st->device_ms = double(c.t_end - c.t_begin) * 1e-5;
     st->host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     st->bu_level_mask = c.bu_mask;
     st->grid_blocks = g->last_grid;
@@ -1695,6 +1771,7 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
     st->examined_edges = -1;
     if (g->cfg.count_examined) {
       unsigned long long ex[kStripes * 4];
+      HIP_TRY(hipStreamSynchronize(g->stream));
       HIP_TRY(hipMemcpy(ex, g->ctl->exam, sizeof ex, hipMemcpyDeviceToHost));
       unsigned long long t[3] = {0, 0, 0};
       for (int k = 0; k < kStripes; ++k)
@@ -1703,6 +1780,18 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
       st->examined_bytes = int64_t(8 * t[0] + 8 * t[1] + t[2]);
     }
   }
+  return ENT_OK;
+}
+
+int ent_last_device_ms(ent_graph *g, double *ms) {
+  clear_error();
+  if (!g || !ms) return set_error(ENT_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> lk(g->mu);
+  HIP_TRY(hipSetDevice(g->device));
+  HIP_TRY(hipEventSynchronize(g->ev1));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, g->ev0, g->ev1));
+  *ms = t;
   return ENT_OK;
 }
 

@@ -75,8 +75,10 @@ struct ent_graph {
   int *qbuf[2] = {nullptr, nullptr};  // striped frontier queues (kQStripes x qcap_s)
   int qcap_s = 0;
   ent::Control *ctl = nullptr;
-  ent::Control *h_ctl = nullptr;     // host-coherent; the kernels mirror their state into h_ctl->st
-  ent::Control *h_st_dev = nullptr;  // the same memory as a device pointer
+  ent::Control *h_ctl = nullptr;           // pinned; the persistent form's state copy
+  unsigned long long *h_mir = nullptr;     // host-coherent tagged state mirror, written by the kernels
+  unsigned long long *h_mir_dev = nullptr; // the same memory as a device pointer
+  unsigned search_seq = 0;                 // number of the latest search on this handle (never 0)
   ent_bfs_config cfg{};
   int cus = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -103,7 +103,8 @@ typedef struct ent_stats {
                                the reference's TEPS numerator
                                (validate.h:115-123) */
   int64_t alg_bytes;        /* 8 M_c + 12 N_c + 4 n */
-  double device_ms;         /* device time of the search (HIP events) */
+  double device_ms;         /* device time of the search: constant 100 MHz clock from
+                               k_init to the launch that found it finished */
   double host_ms;           /* wall time of the call */
   uint64_t bu_level_mask;   /* bit L set: level L ran bottom-up (L < 64) */
   int32_t grid_blocks;      /* blocks per level launch / persistent grid */
@@ -168,6 +169,15 @@ int ent_bfs_validate_device(ent_graph *g, int32_t root, const int32_t *depth_dev
  * entries, either may be NULL) receive both arrays. */
 int ent_bfs_parents(ent_graph *g, int32_t root, int32_t *depth_host,
                     int32_t *parent_host, ent_stats *st);
+
+/* Device time of the last search on g from HIP events on the handle's stream
+ * (recorded before its first launch and behind its last queued launch),
+ * waiting for those launches to drain. ent_bfs_device returns as soon as the
+ * search is finished, and its stats.device_ms comes from the device's
+ * constant clock; this is the event-timed figure of the same search. The
+ * reference has no device timer: it times each root with host wall time
+ * (bfs_gpu_opt.cuh:381, :411). */
+int ent_last_device_ms(ent_graph *g, double *ms);
 
 /* One launch of the last search run with count_examined = 1 (the analogue
  * of the reference's ENABLE_MONITORING per-level prints,

@@ -296,3 +296,48 @@ def test_level_trace(oracle, rmat16):
         assert dg.level_trace() == []
     finally:
         dg.set_config(base)
+
+
+def test_results_final_on_return(oracle, rmat16):
+    """ent_bfs_device returns once the search is finished, while launches
+    queued behind it may still run: the depth array read at once through
+    another stream (a synchronous hipMemcpy) is already the final one, and the
+    device times are consistent."""
+    import ctypes as C
+    g, dg = rmat16
+    base = dg.config
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    try:
+        for launch in (L.LAUNCH_STREAM, L.LAUNCH_LEVELS):
+            dg.configure(launch=launch)
+            for r in g.pick_roots(6, 31):
+                r = int(r)
+                st = dg.bfs_device(r)
+                got = np.empty(g.n, dtype=np.int32)
+                assert hip.hipMemcpy(got.ctypes.data, dg.depth_device_ptr(), 4 * g.n, 2) == 0
+                assert np.array_equal(got, oracle.bfs(g.beg_pos, g.adj, r)), (launch, r)
+                ev = dg.last_device_ms()
+                assert 0 < st["device_ms"] <= ev * 1.2 + 0.05, (st["device_ms"], ev)
+    finally:
+        dg.set_config(base)
+
+
+def test_back_to_back_searches(oracle, rmat16):
+    """Searches on one handle that follow each other at once, every one with
+    another launch form, direction and root: the host's wait on a search must
+    never take the previous search's final state (its mirrored state words
+    are overwritten while the host reads them) for the current one."""
+    g, dg = rmat16
+    base = dg.config
+    roots = [int(r) for r in g.pick_roots(48, 77)]
+    forms = list(LAUNCHES.items())
+    dirs = list(DIRS.items())
+    try:
+        for k, r in enumerate(roots):
+            lname, launch = forms[k % len(forms)]
+            dname, d = dirs[(k // len(forms) + k) % len(dirs)]
+            dg.configure(direction=d, launch=launch)
+            _check(oracle, g, dg, r, f"rmat16/back-to-back {k}/{dname}/{lname}")
+    finally:
+        dg.set_config(base)
