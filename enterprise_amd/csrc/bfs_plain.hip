@@ -896,10 +896,26 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu, int search) {
   const long long gt = (long long)blockIdx.x * BS + threadIdx.x;
   const long long nt = (long long)gridDim.x * BS;
-  for (long long v = gt; v < a.n; v += nt) {
-    a.depth[v] = v == root ? 0 : -1;
-    a.mark[v] = 0;
+  // Depths in 16-byte stores from the array's first 16-byte boundary (a
+  // caller's array need only be 4-byte aligned); the entries before it and
+  // after the last full group are written one by one (at most 6).
+  const long long head = (long long)((0ull - reinterpret_cast<unsigned long long>(a.depth)) & 15ull) >> 2;
+  const long long h = head < a.n ? head : (long long)a.n;
+  const long long nq = ((long long)a.n - h) >> 2;
+  int4 *d4 = reinterpret_cast<int4 *>(a.depth + h);
+  for (long long q = gt; q < nq; q += nt) {
+    const long long r = (long long)root - (h + (q << 2));
+    d4[q] = make_int4(r == 0 ? 0 : -1, r == 1 ? 0 : -1, r == 2 ? 0 : -1, r == 3 ? 0 : -1);
   }
+  const long long t0 = h + (nq << 2);
+  if (gt < h + ((long long)a.n - t0)) {
+    const long long v = gt < h ? gt : t0 + (gt - h);
+    a.depth[v] = v == root ? 0 : -1;
+  }
+  // Marks (the library's own allocation, 256-byte aligned) in 16-byte stores.
+  const long long nm = (long long)a.n >> 4;
+  for (long long q = gt; q < nm; q += nt) reinterpret_cast<uint4 *>(a.mark)[q] = make_uint4(0u, 0u, 0u, 0u);
+  if (gt < (long long)a.n - (nm << 4)) a.mark[(nm << 4) + gt] = 0;
   for (long long w = gt; w < a.nwords; w += nt) {
     const unsigned long long rb = (w == (root >> 6)) ? 1ull << (root & 63) : 0ull;
     a.visited[w] = ~a.nzmask[w] | rb;

@@ -341,3 +341,31 @@ def test_back_to_back_searches(oracle, rmat16):
             _check(oracle, g, dg, r, f"rmat16/back-to-back {k}/{dname}/{lname}")
     finally:
         dg.set_config(base)
+
+
+@pytest.mark.parametrize("n", [3, 5, 1021, 1022, 1023, 1024, 1037])
+def test_unaligned_depth_arrays(oracle, n):
+    """ent_bfs_device into a caller's depth array at every 4-byte offset from
+    a 16-byte boundary, on graphs whose vertex count is not a multiple of 4
+    or 16: the depths are exact and the words either side stay untouched."""
+    import ctypes as C
+    rng = np.random.default_rng(n)
+    src = rng.integers(0, n, size=4 * n)
+    dst = rng.integers(0, n, size=4 * n)
+    g = HostGraph.from_edges(n, src, dst)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    guard = np.frombuffer(b"\x7f" * 4, dtype=np.int32)[0]
+    with DeviceGraph(g) as dg:
+        base = dg.device_alloc(4 * (n + 8))
+        try:
+            for k in range(4):
+                assert hip.hipMemset(base, 0x7F, 4 * (n + 8)) == 0
+                root = (7 * k + 1) % n
+                _say(f"[gpu] unaligned depth n={n} offset={4 * k} root={root}")
+                dg.bfs_device(root, base + 4 * k)
+                got = dg.copy_depth(base, n + 8)
+                assert np.array_equal(got[k:k + n], oracle.bfs(g.beg_pos, g.adj, root)), (n, k, root)
+                assert (got[:k] == guard).all() and (got[k + n:] == guard).all(), (n, k)
+        finally:
+            dg.device_free(base)
