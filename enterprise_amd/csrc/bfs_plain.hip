@@ -54,6 +54,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "ent_bfs.h"
@@ -1339,8 +1340,8 @@ int poll_state(ent_graph *g, long long last, State *out) {
 // its state, and returns that state. The launch that finds the search
 // finished starts only after every earlier launch has completed, so the
 // results are final; launches queued behind it (they exit at once) may still
-// be running on the stream. The stream is queried now and then, so an error
-// or a drained stream ends the wait too.
+// be running on the stream. The stream is queried every 256 spins, so an
+// error or a drained stream ends the wait too.
 int wait_state(ent_graph *g, long long last, unsigned search, State *out) {
   if (last >= kSpinMaxLaunch) return poll_state(g, last, out);
   const unsigned want = unsigned(last) + search * kTagMul;
@@ -1359,6 +1360,12 @@ int wait_state(ent_graph *g, long long last, unsigned search, State *out) {
       if (e == hipSuccess) return poll_state(g, last, out);  // drained: the mirror is complete
       if (e != hipErrorNotReady) return set_error(ENT_EDEVICE, "search failed: %s", hipGetErrorString(e));
     }
+    // A pause hint per spin; past ~16K spins (milliseconds) the thread also
+    // yields, so long searches do not hold a host core to themselves.
+#if defined(__x86_64__)
+    __builtin_ia32_pause();
+#endif
+    if (spins >= (1ul << 14)) std::this_thread::yield();
   }
 }
 
