@@ -1413,9 +1413,13 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
     case ENT_LAUNCH_LEVELS:
     case ENT_LAUNCH_STREAM: {
       // First batch: as many launches as the previous search on this handle
-      // needed (one host round trip in the common case), then small batches.
+      // needed, the one that found it finished included. Launches beyond the
+      // finish still run (they exit at once, ~4 us each), so the batch is not
+      // padded; a search that needs more gets follow-up batches of 1, 2, 4 ...
+      // launches, queued when the last queued launch has started.
       long long i = 0;
       int k = cfg.launch == ENT_LAUNCH_LEVELS ? 1 : std::max(2, g->expect_launches);
+      int follow = 1;
       for (;;) {
         HIP_TRY(queue_phases(g, ks, &a, i, k));
         HIP_TRY(hipEventRecord(g->ev1, g->stream));  // end of the search once it is done
@@ -1424,10 +1428,12 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
         if (rc) return rc;
         if (fin->done) break;
         if (i > max_launches) return set_error(ENT_EINTERNAL, "search did not finish in %lld launches", i);
-        // Follow-up batches double (deep searches need thousands of phases).
-        if (cfg.launch == ENT_LAUNCH_STREAM) k = std::min(2 * k, 1024);
+        if (cfg.launch == ENT_LAUNCH_STREAM) {
+          k = follow;
+          follow = std::min(2 * follow, 1024);
+        }
       }
-      if (cfg.launch == ENT_LAUNCH_STREAM) g->expect_launches = int(std::min<int64_t>(fin->phases + 1, 1 << 16));
+      if (cfg.launch == ENT_LAUNCH_STREAM) g->expect_launches = int(std::min<int64_t>(fin->phases, 1 << 16));
       return ENT_OK;
     }
     case ENT_LAUNCH_GRAPH: {
