@@ -221,7 +221,10 @@ int ent_get_config(const ent_graph *g, ent_bfs_config *cfg);
 int ent_bfs(ent_graph *g, int32_t root, int32_t *depth_host, ent_stats *stats);
 
 /* Same, writing the n-element int32 depth array to device memory depth_dev
- * (NULL = the handle's own buffer) and returning once the search is done. */
+ * (NULL = the handle's own buffer; 4-byte alignment suffices) and returning
+ * once the search is done: the depths are final and may be read through any
+ * stream, while launches that exit at once may still be queued on the
+ * handle's stream (later calls on the handle are ordered behind them). */
 int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev,
                    ent_stats *stats);
 
