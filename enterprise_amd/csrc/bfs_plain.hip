@@ -107,11 +107,15 @@ struct State {
 // qn), mirrored by block 0 into host memory after every launch.
 constexpr int kHostStateWords = int(offsetof(State, qn) / 4);
 // The mirror holds each word in the low half of a 64-bit word whose high half
-// is the tag of the launch that wrote it: launch + search * kTagMul (mod
-// 2^32). The lanes' stores reach host memory in no fixed order, so the host
-// takes a slot only when all its tags agree. Tags of two launches of one
-// search, or of consecutive searches, differ while launch indices stay below
-// kSpinMaxLaunch (the host polls after a stream synchronisation beyond it).
+// is the tag of the launch that wrote it: phases + search * kTagMul (mod
+// 2^32), where phases (launches run so far, this one included) is the launch
+// index + 1 until the search is finished and stays fixed after it (the
+// launches after the finish publish identical states). Unlike the kernel's
+// launch argument it also counts across hipGraph replays. The lanes' stores
+// reach host memory in no fixed order, so the host takes a slot only when all
+// its tags agree. Tags of two launches of one search, or of consecutive
+// searches, differ while searches stay below kSpinMaxLaunch launches (the host
+// polls after a stream synchronisation beyond it).
 constexpr unsigned kTagMul = 0x9E3779B1u;
 constexpr long long kSpinMaxLaunch = 1ll << 24;
 
@@ -878,7 +882,7 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
       for (int w = lane; w < int(sizeof(State) / 4); w += kWave) st_agent(d32 + w, s32[w]);
       // Tagged host copy: the host reads it without a device-to-host copy.
       if (lane < kHostStateWords) {
-        const unsigned tag = unsigned(i) + unsigned(sm.S.search) * kTagMul;
+        const unsigned tag = unsigned(sm.S.phases) + unsigned(sm.S.search) * kTagMul;
         a.hmir[(i & 1) * kHostStateWords + lane] = (static_cast<unsigned long long>(tag) << 32) | s32[lane];
       }
     }
@@ -1344,7 +1348,7 @@ int poll_state(ent_graph *g, long long last, State *out) {
 // error or a drained stream ends the wait too.
 int wait_state(ent_graph *g, long long last, unsigned search, State *out) {
   if (last >= kSpinMaxLaunch) return poll_state(g, last, out);
-  const unsigned want = unsigned(last) + search * kTagMul;
+  const unsigned want = unsigned(last + 1) + search * kTagMul;
   State s;
   unsigned tag;
   for (unsigned long spins = 0;; ++spins) {
@@ -1461,10 +1465,12 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
         g->graph_k = k;
       }
       g->last_grid = grid;
+      // Replay r runs launches r * k .. r * k + k - 1 of the search; the next
+      // replay is queued once the last of them has started.
       for (long long i = 0;; i += k) {
         HIP_TRY(hipGraphLaunch(g->gexec, g->stream));
         HIP_TRY(hipEventRecord(g->ev1, g->stream));
-        rc = poll_state(g, k - 1, fin);
+        rc = wait_state(g, i + k - 1, g->search_seq, fin);
         if (rc) return rc;
         if (fin->done) break;
         if (i > max_launches) return set_error(ENT_EINTERNAL, "search did not finish in %lld launches", i);
