@@ -101,6 +101,7 @@ struct State {
   int q_valid;   // frontier of `level` is in the striped queue qbuf[level & 1]
   int pending;   // phase to run after PH_REBUILD
   int search;    // the host's search number, carried from launch to launch
+  int fz;        // bit p: front[p] is all zero (a queue-driven level may OR its output into it)
   unsigned int qn[kStripes];  // entries per queue stripe
 };
 // Leading words of State the host reads after a search (everything before
@@ -232,9 +233,12 @@ __device__ __forceinline__ unsigned deg_max(unsigned md, long long deg) {
 }
 
 // Adds a block's frontier counts to its stripe and appends its staged queue
-// entries to queue stripe (block % 64) of parity `par`.
+// entries to queue stripe (block % 64) of parity `par`; with bm, also ORs
+// every staged vertex into that bitmap (complete unless the stage overflowed,
+// which marks the queue incomplete too).
 template <int BS>
-__device__ __forceinline__ void add_counts(const Args &a, Smem<BS> &sm, int slot, int par) {
+__device__ __forceinline__ void add_counts(const Args &a, Smem<BS> &sm, int slot, int par,
+                                           unsigned long long *bm = nullptr) {
   __syncthreads();
   unsigned long long *c = a.ctl->cnt[slot].v + (blockIdx.x % kStripes) * kStripeU64;
   if (threadIdx.x == 0) {
@@ -253,8 +257,11 @@ __device__ __forceinline__ void add_counts(const Args &a, Smem<BS> &sm, int slot
   const int k = min(sm.qcount, kQStage);
   const int base = sm.qbase;
   int *dst = a.qbuf[par] + (long long)(blockIdx.x % kStripes) * a.qcap_s;
-  for (int j = threadIdx.x; j < k; j += BS)
-    if (base + j < a.qcap_s) dst[base + j] = sm.qstage[j];
+  for (int j = threadIdx.x; j < k; j += BS) {
+    const int v = sm.qstage[j];
+    if (base + j < a.qcap_s) dst[base + j] = v;
+    if (bm) atomicOr(bm + (v >> 6), 1ull << (v & 63));
+  }
 }
 
 // Sums a per-lane count over the wavefront and adds it to the block total.
@@ -614,6 +621,36 @@ __device__ void bu_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<COU
   add_counts(a, sm, slot, (L + 1) & 1);
 }
 
+// Whether the queue-driven level L (state s) also ORs the vertices it claims
+// into front[(L + 1) & 1] (only when that bitmap is all zero), and whether it
+// zeroes front[L & 1], which no later phase reads, by clearing the words of
+// its queue's vertices (only when the bitmap holds exactly that frontier).
+// The bitmap spares the rebuild sweep when a small frontier grows into one
+// that a bitmap-driven level expands, which happens in a search's first
+// levels; later queue-driven levels (thousands of them on a high-diameter
+// graph, each a short latency chain) skip it, and once one has, the clearing
+// stops too. Both are off with the LDS hub table, whose frontier hubs the
+// rebuild records.
+constexpr int kTdqBitmapLevels = 3;
+__device__ __forceinline__ bool tdq_writes_bitmap(const Args &a, const State &s, int L) {
+  return a.hub_mask < 0 && L < kTdqBitmapLevels && ((s.fz >> ((L + 1) & 1)) & 1);
+}
+__device__ __forceinline__ bool tdq_clears_bitmap(const Args &a, const State &s) {
+  return a.hub_mask < 0 && s.bm_valid;
+}
+
+// Queue entry k of the striped queue qin (stripe prefix sums in sm.qpre).
+template <int BS>
+__device__ __forceinline__ int queue_entry(const Args &a, const Smem<BS> &sm, const int *qin, long long k) {
+  int lo = 0, hi = kStripes - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sm.qpre[mid] <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  return qin[(long long)lo * a.qcap_s + (k - sm.qpre[lo])];
+}
+
 // Top-down level L from the striped queue (small frontiers whose vertices all
 // have degree < queue_max_degree): one launch per level. A lane (or, for degree >=
 // t_wave, the whole wavefront) expands a frontier vertex; each unvisited
@@ -631,6 +668,14 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<CO
   __syncthreads();
   const int total = sm.qpre[kStripes];
   const int *qin = a.qbuf[L & 1];
+  if (tdq_clears_bitmap(a, sm.S)) {
+    // Every bit of a cleared word belongs to this frontier.
+    unsigned long long *fc = a.front[L & 1];
+    for (long long k = (long long)blockIdx.x * BS + threadIdx.x; k < total; k += (long long)gridDim.x * BS) {
+      fc[queue_entry(a, sm, qin, k) >> 6] = 0ull;
+      tl.s(8);
+    }
+  }
   unsigned long long nf = 0, mf = 0;
   unsigned md = 0;
   // Claims the unvisited among adj[i], i = i0, i0 + stride, ... < e, four
@@ -704,7 +749,7 @@ __device__ void tdq_phase(const Args &a, Smem<BS> &sm, int L, int slot, Tally<CO
     }
   }
   wave_add_counts(sm, nf, mf, md);
-  add_counts(a, sm, slot, (L + 1) & 1);
+  add_counts(a, sm, slot, (L + 1) & 1, tdq_writes_bitmap(a, sm.S, L) ? a.front[(L + 1) & 1] : nullptr);
 }
 
 // Rebuilds front[L & 1] from the depth array (depth == L), one wavefront per
@@ -742,6 +787,7 @@ __device__ __forceinline__ void next_state(const Args &a, State &s, const unsign
   if (pphase == PH_REBUILD) {
     s.phase = s.pending;
     s.bm_valid = 1;
+    s.fz &= ~(1 << (s.level & 1));
     return;
   }
   const unsigned long long nf = tot[K_NF], mf = tot[K_MF];
@@ -782,11 +828,18 @@ __device__ __forceinline__ void next_state(const Args &a, State &s, const unsign
       else next_bu = !(nf * (unsigned long long)a.beta < (unsigned long long)a.n_active && nf < s.prev_nf);
     }
   }
-  // Frontier L + 1 is a bitmap after an inspection or a bottom-up level, and
-  // a complete queue (from any producer) unless a block or stripe overflowed.
-  const bool bm = pphase == PH_INSPECT || pphase == PH_BU;
+  // Frontier L + 1 is a complete queue (from any producer) unless a block or
+  // stripe overflowed, and a bitmap after an inspection or a bottom-up level,
+  // or after a queue-driven level that found its bitmap all zero (it ORs its
+  // claims into it; complete when the queue is). A queue-driven level clears
+  // its own frontier's bitmap, so consecutive queue-driven levels keep one.
   bool q = pphase == PH_INSPECT || pphase == PH_TDQ || pphase == PH_BU;
   if (!q_fit || tot[K_QOVF]) q = false;
+  const int nb = (L + 1) & 1;
+  const bool tdq_w = pphase == PH_TDQ && tdq_writes_bitmap(a, s, L);
+  if (pphase == PH_INSPECT || pphase == PH_BU || tdq_w) s.fz &= ~(1 << nb);
+  if (pphase == PH_TDQ && tdq_clears_bitmap(a, s)) s.fz |= 1 << (L & 1);
+  const bool bm = pphase == PH_INSPECT || pphase == PH_BU || (tdq_w && q);
   s.prev_nf = nf;
   s.level = L + 1;
   s.bm_valid = bm ? 1 : 0;
@@ -925,6 +978,7 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu, int
     const unsigned long long rb = (w == (root >> 6)) ? 1ull << (root & 63) : 0ull;
     a.visited[w] = ~a.nzmask[w] | rb;
     a.front[0][w] = rb;
+    a.front[1][w] = 0ull;
   }
   Control *c = a.ctl;
   constexpr long long kC = (long long)kStripes * kStripeU64;
@@ -945,6 +999,7 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu, int
     S.level = -1;
     S.phase = first_bu ? PH_BU : PH_INSPECT;
     S.search = search;
+    S.fz = 2;
     S.t_begin = rt_clock();
     c->st[1] = S;
     c->st[0] = S;
