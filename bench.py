@@ -1,18 +1,26 @@
 """Benchmark of the MI355X direction-optimizing BFS.
 
-Default workload (one step = one single-source BFS over the resident graph):
-R-MAT scale 22, edgefactor 16 (A/B/C/D = .57/.19/.19/.05, seed 22, relabelled,
-symmetrized, loops and duplicates removed), 64 seeded non-isolated roots
-(seed 1002) run one at a time. Each step is timed on the host around the C-ABI
-call (the reference's window, bfs_gpu_opt.cuh:381-411: from the call to the
-end of the search, depth array left in HBM). The reported value is the
-harmonic mean of per-root GTEPS with the reference's numerator M_c = sum of
-degrees over reached vertices (validate.h:115-123).
+Default workload: R-MAT scale 22, edgefactor 16 (A/B/C/D = .57/.19/.19/.05,
+seed 22, relabelled, symmetrized, loops and duplicates removed), 64 seeded
+non-isolated roots (seed 1002) run one at a time on the resident graph. One
+step is one sweep over all 64 roots; --steps K times K sweeps. Each search is
+timed on the host around the C-ABI call (the reference's window,
+bfs_gpu_opt.cuh:381-411: from the call to the end of the search, depth array
+left in HBM). A root's time is its mean over the K sweeps; the reported value
+is the harmonic mean over the roots of per-root GTEPS with the reference's
+numerator M_c = sum of degrees over reached vertices (validate.h:115-123),
+counting, as the reference does, the roots whose search goes deeper than two
+levels (bfs_gpu_opt.cuh:413-445).
 
-Other modes (--mode): grid (4096^2 high-diameter grid, per-level cost of the
-launch forms), batch (512-source multi-source BFS on R-MAT scale 23), serve
-(continuous-batching serving loop) and csr (CSR construction on the device
-from the raw tuples of R-MAT scale --scale); each prints its own JSON line.
+The default line also carries, under "extra", the 512-source multi-source
+search on R-MAT scale 23 against the plain path on the same roots
+(extra.batch) and the per-level cost of each launch form on the 4096^2
+high-diameter grid (extra.grid), each with its oracle parity and CPU figures
+(--no-extra skips both).
+
+Other modes (--mode): grid, batch, serve (continuous-batching serving loop)
+and csr (CSR construction on the device from the raw tuples of R-MAT scale
+--scale); each prints its own JSON line.
 """
 import argparse
 import json
@@ -83,14 +91,15 @@ def grid_cpu_check(g, checks):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ct
     oracle = oracle_ct.Oracle()
-    cpu_ms, mism = [], 0
-    for r, got in checks:
+    cpu_ms, mism, forms = [], 0, []
+    for r, got, form in checks:
         t0 = time.perf_counter()
         want = oracle.bfs(g.beg_pos, g.adj, int(r))
         cpu_ms.append((time.perf_counter() - t0) * 1e3)
         mism += int(not np.array_equal(got, want))
+        forms.append(form)
     return {"cpu_serial_ms_per_bfs": round(float(np.mean(cpu_ms)), 1), "parity_roots": len(checks),
-            "parity_mismatches": mism}
+            "parity_forms": forms, "parity_mismatches": mism}
 
 
 def csr_cpu_baseline(n, src, dst, sample=64 << 20):
@@ -148,6 +157,46 @@ def load_tuned(dg, name):
     return None
 
 
+def batch_cpu_baseline(g, roots, budget_s, threads):
+    """Batch mode: the OpenMP direction-optimizing oracle BFS on `threads`
+    host cores over a bounded prefix of the same 512 roots (one at a time),
+    as aggregate GTEPS (sum of M_c over sum of times)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ct
+    oracle = oracle_ct.Oracle()
+    edges_sum, t_sum, done = 0, 0.0, 0
+    for r in roots:
+        t0 = time.perf_counter()
+        d = oracle.bfs_do(g.beg_pos, g.adj, int(r), threads=threads)
+        t_sum += time.perf_counter() - t0
+        edges_sum += oracle.report(g.beg_pos, d)[0]
+        done += 1
+        if t_sum > budget_s:
+            break
+    return {"value": round(edges_sum / t_sum / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"OpenMP direction-optimizing BFS (oracle) on {threads} threads, the first {done} of the "
+                      f"{len(roots)} roots one at a time, aggregate; host {cpu_model()}"}
+
+
+def run_extra(args, mode, **over):
+    """Runs benchmarks mode `mode` with argument overrides and its CPU legs
+    (oracle parity and baseline, which stay in this file)."""
+    from enterprise_amd import benchmarks
+    ns = argparse.Namespace(**vars(args))
+    for k, v in over.items():
+        setattr(ns, k, v)
+    ns.mode = mode
+    out = benchmarks.run(ns)
+    chk = out.pop("_cpu_check", None)
+    if chk is not None:
+        out.update(grid_cpu_check(chk["graph"], chk["checks"]))
+    cb = out.pop("_cpu_batch", None)
+    if cb is not None:
+        out["cpu_baseline"] = None if args.no_cpu else batch_cpu_baseline(cb["graph"], cb["roots"],
+                                                                           args.cpu_budget / 2, host_threads())
+    return out
+
+
 def bench_plain(args):
     from enterprise_amd import DeviceGraph, HostGraph, traffic
     from enterprise_amd import _lib as L
@@ -168,57 +217,62 @@ def bench_plain(args):
         dg.configure(launch={"persistent": L.LAUNCH_PERSISTENT, "levels": L.LAUNCH_LEVELS,
                              "graph": L.LAUNCH_GRAPH, "stream": L.LAUNCH_STREAM}[args.launch])
     log(f"[bench] config {dg.config.as_dict()} (tuned file: {tuned})")
-    for i in range(args.warmup):
-        dg.bfs_device(int(roots[i % 64]))
+    for _ in range(args.warmup):              # one warm-up step = one sweep over the roots
+        for r in roots:
+            dg.bfs_device(int(r))
     log("[bench] warmup done")
-    per = []
-    gpu_depths = {}
-    dg_sync = L.lib()
+    host_s = [[] for _ in roots]              # per root: host window of each sweep
+    ev_ms = [[] for _ in roots]               # per root: HIP-event device time of each sweep
+    stats = [None] * len(roots)
     t_all0 = time.perf_counter()
-    for k in range(args.steps):
-        r = int(roots[k % 64])
-        t0 = time.perf_counter()
-        st = dg.bfs_device(r)
-        dt = time.perf_counter() - t0
-        # HIP-event device time of the same search, read after the host
-        # window (it waits for the launches queued behind the finished search).
-        st["event_ms"] = dg.last_device_ms()
-        per.append((r, dt, st))
+    for _ in range(args.steps):
+        for i, r in enumerate(roots):
+            t0 = time.perf_counter()
+            st = dg.bfs_device(int(r))
+            host_s[i].append(time.perf_counter() - t0)
+            # HIP-event device time of the same search, read after the host
+            # window (it waits for the launches queued behind the finished
+            # search).
+            ev_ms[i].append(dg.last_device_ms())
+            stats[i] = st
     t_all = time.perf_counter() - t_all0
-    del dg_sync
-    log(f"[bench] timed {args.steps} BFS in {t_all:.3f}s")
+    log(f"[bench] timed {args.steps} sweeps of {len(roots)} roots in {t_all:.3f}s")
     # Depths of a few timed roots for the CPU check (outside the timed region);
     # these calls copy the depth array to host memory, so their wall time is
     # the PCIe-inclusive rate.
     pcie = []
+    gpu_depths = {}
     for i in range(min(args.check_roots, 64)):
         t0 = time.perf_counter()
         d, st = dg.bfs(int(roots[i]))
         pcie.append(st["reached_edges"] / (time.perf_counter() - t0) / 1e9)
         gpu_depths[i] = d
-    # Schedule-exact examined bytes: the counting kernel variant over the
-    # same roots (outside the timed region; it is slower).
     # Every timed root once more, each result checked on the device for all
     # vertices (ent_bfs_validate_device; outside the timed region).
     violations = 0
     for r in roots:
         dg.bfs_device(int(r))
         violations += sum(dg.validate_device(int(r)))
+    # Schedule-exact examined bytes: the counting kernel variant over the
+    # same roots (outside the timed region; it is slower).
     exam = []
     if not args.no_examined:
         dg.configure(count_examined=1)
         exam = [dg.bfs_device(int(r))["examined_bytes"] for r in roots]
         dg.configure(count_examined=0)
-    used = [(r, dt, st) for (r, dt, st) in per if st["levels"] - 1 > 2]
-    gteps = [st["reached_edges"] / dt / 1e9 for (r, dt, st) in used]
-    gteps_dev = [st["reached_edges"] / (st["event_ms"] * 1e-3) / 1e9 for (r, dt, st) in used]
-    alg_bytes = [st["alg_bytes"] for (_, _, st) in used]
-    dev_ms = [st["event_ms"] for (_, _, st) in used]
-    clock_ms = [st["device_ms"] for (_, _, st) in used]
-    host_ms = [dt * 1e3 for (_, dt, _) in used]
-    achieved = sum(alg_bytes) / (sum(dev_ms) * 1e-3) / 1e9  # GB/s over HIP-event device time
-    achieved_host = sum(alg_bytes) / (sum(host_ms) * 1e-3) / 1e9
+    used = [i for i in range(len(roots)) if stats[i]["levels"] - 1 > 2]
+    mean_s = {i: float(np.mean(host_s[i])) for i in used}
+    gteps = [stats[i]["reached_edges"] / mean_s[i] / 1e9 for i in used]
+    gteps_dev = [stats[i]["reached_edges"] / (float(np.mean(ev_ms[i])) * 1e-3) / 1e9 for i in used]
+    alg_bytes = [stats[i]["alg_bytes"] for i in used]
+    # Algorithmic bytes over HIP-event device time, every timed search of the
+    # counted roots.
+    dev_sum_s = sum(sum(ev_ms[i]) for i in used) * 1e-3
+    achieved = sum(a * args.steps for a in alg_bytes) / dev_sum_s / 1e9
+    host_sum_s = sum(sum(host_s[i]) for i in used)
+    achieved_host = sum(a * args.steps for a in alg_bytes) / host_sum_s / 1e9
     value = hmean(gteps)
+    all_host_ms = [x * 1e3 for i in used for x in host_s[i]]
     cpu = None
     cpu_serial = None
     parity = None
@@ -233,15 +287,17 @@ def bench_plain(args):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(float(np.mean(host_ms)), 4),
+        "ms_per_step": round(t_all * 1e3 / max(1, args.steps), 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": f"plain single-source DO-BFS, R-MAT scale {scale} ef 16 "
-                               f"(a=.57 b=.19 c=.19, seed {scale}), 64 roots (seed 1002)",
-                   "n": g.n, "m_directed": g.m, "roots": 64,
+                               f"(a=.57 b=.19 c=.19, seed {scale}), 64 roots (seed 1002); "
+                               "one step = one sweep over the 64 roots",
+                   "n": g.n, "m_directed": g.m, "roots": len(roots), "roots_in_mean": len(used),
+                   "sweeps": args.steps,
                    "launch": dg.config.launch, "direction": dg.config.direction,
                    "rows": "caller order" if args.keep_order else "by neighbour degree class"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
@@ -258,12 +314,12 @@ def bench_plain(args):
             "gteps_amean": round(float(np.mean(gteps)), 3),
             "gteps_graph500_undirected_hmean": round(value / 2, 3),
             "alg_GBps_host_window": round(achieved_host, 1),
-            "device_ms_mean": round(float(np.mean(dev_ms)), 4),
-            "device_clock_ms_mean": round(float(np.mean(clock_ms)), 4),
-            "host_ms_p50": round(float(np.percentile(host_ms, 50)), 4),
-            "levels_mean": round(float(np.mean([st["levels"] for (_, _, st) in used])), 2),
+            "ms_per_bfs_host_window_mean": round(float(np.mean(all_host_ms)), 4),
+            "device_ms_mean": round(float(np.mean([x for i in used for x in ev_ms[i]])), 4),
+            "device_clock_ms_mean": round(float(np.mean([stats[i]["device_ms"] for i in used])), 4),
+            "host_ms_p50": round(float(np.percentile(all_host_ms, 50)), 4),
+            "levels_mean": round(float(np.mean([stats[i]["levels"] for i in used])), 2),
             "upload_s": round(upload_s, 3),
-            "roots_used": len(used),
             "parity": parity,
             "device_validation": {"roots": len(roots), "violations": int(violations)},
             "cpu_serial_1core": cpu_serial,
@@ -271,14 +327,32 @@ def bench_plain(args):
         },
     }
     dg.free()
+    del g
+    if not args.no_extra:
+        log("[bench] extra: 512-source batch on R-MAT scale 23")
+        b = run_extra(args, "batch", scale=23, steps=3, warmup=1)
+        out["extra"]["batch"] = {k: b[k] for k in (
+            "value", "unit", "batch_ms", "batch_device_ms", "plain_aggregate_gteps", "speedup_vs_plain",
+            "rows_mismatching_plain", "levels", "fallback_rows", "roofline", "cpu_baseline", "config")}
+        log(f"[bench] batch: {out['extra']['batch']['speedup_vs_plain']}x the plain path")
+        log("[bench] extra: 4096^2 grid, launch forms")
+        gr = run_extra(args, "grid", grid_roots=0.25, set=[])
+        out["extra"]["grid"] = {k: gr[k] for k in (
+            "value", "unit", "best_form", "forms", "device_validation", "cpu_serial_ms_per_bfs",
+            "parity_roots", "parity_forms", "parity_mismatches", "config") if k in gr}
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=10,
+                    help="plain mode: sweeps over the 64 roots (one step = one sweep)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="plain mode: skip the batch and grid figures after the plain line")
+    ap.add_argument("--grid-roots", type=float, default=1.0,
+                    help="grid mode: fraction of the roots timed per launch form")
     ap.add_argument("--mode", default="plain", choices=["plain", "grid", "batch", "serve", "calib", "csr"])
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--launch", default=None, choices=["persistent", "levels", "graph", "stream"])
@@ -313,10 +387,10 @@ def main():
     if args.traffic:
         from enterprise_amd import traffic
         if args.mode == "plain":
-            child = ["--no-cpu", "--no-examined", "--steps", str(args.steps), "--warmup", "0", "--check-roots", "0",
-                     "--scale", str(args.scale)]
+            child = ["--no-cpu", "--no-examined", "--no-extra", "--steps", "1", "--warmup", "0",
+                     "--check-roots", "0", "--scale", str(args.scale)]
         elif args.mode == "batch":
-            child = ["--mode", "batch", "--steps", "1", "--warmup", "0"]
+            child = ["--mode", "batch", "--steps", "1", "--warmup", "0", "--no-cpu"]
         else:
             raise SystemExit("--traffic supports --mode plain and batch")
         child += [x for kv in args.set for x in ("--set", kv)]
@@ -328,10 +402,7 @@ def main():
         out = bench_plain(args)
     else:
         from enterprise_amd import benchmarks
-        out = benchmarks.run(args)
-        chk = out.pop("_cpu_check", None)
-        if chk is not None:
-            out.update(grid_cpu_check(chk["graph"], chk["checks"]))
+        out = run_extra(args, args.mode) if args.mode in ("grid", "batch") else benchmarks.run(args)
         csr = out.pop("_cpu_csr", None)
         if csr is not None:
             out["cpu_baseline"] = csr_cpu_baseline(csr["n"], csr["src"], csr["dst"])

@@ -68,20 +68,22 @@ def measure(dg, roots, reps=3):
     return _hmean(list(best.values()))
 
 
-def tune(dg: DeviceGraph, roots, check_roots, rounds=2, log=print, budget_s=120.0):
-    """Returns (best config dict, best score, history)."""
+def tune(dg: DeviceGraph, roots, check_roots, rounds=2, log=print, budget_s=120.0, space=None, reps=3):
+    """Returns (best config dict, best score, history). space: knob -> list
+    of candidate values (default SPACE)."""
+    space = SPACE if space is None else space
     base = dg.config.as_dict()
     want = {int(r): dg.bfs(int(r))[0] for r in check_roots}
     best = dict(base)
     for r in roots[:4]:
         dg.bfs_device(int(r))  # warm up
-    best_score = measure(dg, roots)
+    best_score = measure(dg, roots, reps)
     history = [(dict(best), best_score)]
     log(f"[tune] start {best_score:.1f} GTEPS")
     t_start = time.perf_counter()
     for rnd in range(rounds):
         improved = False
-        for knob, values in SPACE.items():
+        for knob, values in space.items():
             for v in values:
                 if v == best[knob] or time.perf_counter() - t_start > budget_s:
                     continue
@@ -96,7 +98,7 @@ def tune(dg: DeviceGraph, roots, check_roots, rounds=2, log=print, budget_s=120.
                 ok = all(np.array_equal(dg.bfs(r)[0], d) for r, d in want.items())
                 if not ok:
                     raise RuntimeError(f"depth mismatch under config {cand}")
-                score = measure(dg, roots)
+                score = measure(dg, roots, reps)
                 history.append((dict(cand), score))
                 if score > best_score * 1.01:
                     log(f"[tune] {knob}={v}: {score:.1f} GTEPS (was {best_score:.1f})")
@@ -129,10 +131,12 @@ def measure_batch(dg, roots, out_ptr, reps=1):
     return st["reached_edges"] / best / 1e9
 
 
-def tune_batch(dg: DeviceGraph, roots, rounds=2, log=print, budget_s=150.0):
-    """Coordinate descent over BATCH_SPACE for one multi-source search over
-    `roots`, scored by aggregate GTEPS; a candidate must reproduce every row
-    of the starting configuration (device row checksums)."""
+def tune_batch(dg: DeviceGraph, roots, rounds=2, log=print, budget_s=150.0, space=None, reps=3):
+    """Coordinate descent over BATCH_SPACE (or `space`) for one multi-source
+    search over `roots`, scored by aggregate GTEPS; a candidate must
+    reproduce every row of the starting configuration (device row
+    checksums)."""
+    space = BATCH_SPACE if space is None else space
     k = len(roots)
     out = dg.device_alloc(k * dg.n * 4)
     try:
@@ -141,13 +145,13 @@ def tune_batch(dg: DeviceGraph, roots, rounds=2, log=print, budget_s=150.0):
         dg.bfs_batch_device(roots, out)  # warm-up
         want = dg.row_hash(out, k, dg.n).copy()
         best = dict(base)
-        best_score = measure_batch(dg, roots, out, reps=3)
+        best_score = measure_batch(dg, roots, out, reps=reps)
         history = [(dict(best), best_score)]
         log(f"[tune-batch] start {best_score:.1f} GTEPS")
         t_start = time.perf_counter()
         for rnd in range(rounds):
             improved = False
-            for knob, values in BATCH_SPACE.items():
+            for knob, values in space.items():
                 for v in values:
                     if v == best[knob] or time.perf_counter() - t_start > budget_s:
                         continue
@@ -159,7 +163,7 @@ def tune_batch(dg: DeviceGraph, roots, rounds=2, log=print, budget_s=150.0):
                         dg.configure(**cand)
                     except L.EntError:
                         continue
-                    score = measure_batch(dg, roots, out, reps=3)
+                    score = measure_batch(dg, roots, out, reps=reps)
                     if not np.array_equal(dg.row_hash(out, k, dg.n), want):
                         raise RuntimeError(f"row mismatch under config {cand}")
                     history.append((dict(cand), score))
@@ -174,16 +178,17 @@ def tune_batch(dg: DeviceGraph, roots, rounds=2, log=print, budget_s=150.0):
         dg.device_free(out)
 
 
-def save(name, cfg, score, meta):
-    os.makedirs(TUNED_DIR, exist_ok=True)
-    path = os.path.join(TUNED_DIR, f"{name}.json")
+def save(name, cfg, score, meta, out_dir=None):
+    d = TUNED_DIR if out_dir is None else out_dir
+    os.makedirs(d, exist_ok=True)
+    path = os.path.join(d, f"{name}.json")
     with open(path, "w") as f:
         json.dump({"config": cfg, "score_gteps_hmean": round(score, 2), "meta": meta}, f, indent=1)
     return path
 
 
-def load(name):
-    path = os.path.join(TUNED_DIR, f"{name}.json")
+def load(name, tuned_dir=None):
+    path = os.path.join(TUNED_DIR if tuned_dir is None else tuned_dir, f"{name}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -200,6 +205,9 @@ def main():
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--budget", type=float, default=150.0)
     ap.add_argument("--out", default=None, help="output directory for the JSON")
+    ap.add_argument("--seed", type=int, default=None,
+                    help="R-MAT seed of the tuning graph (default: scale + 100, a graph other than "
+                         "the benchmark's, whose seed is the scale)")
     ap.add_argument("--from-tuned", action="store_true",
                     help="start the descent from the saved configuration for this graph, if any")
     ap.add_argument("--batch", type=int, default=0,
@@ -207,14 +215,15 @@ def main():
                          "top-1%% degree vertices) instead of the single-source search")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    g = HostGraph.rmat(args.scale, 16, seed=args.scale)
+    seed = args.scale + 100 if args.seed is None else args.seed
+    g = HostGraph.rmat(args.scale, 16, seed=seed)
     dg = DeviceGraph(g)
     dg.configure(launch=L.LAUNCH_STREAM)
     if args.batch:
         roots = g.pick_roots_mixed(args.batch, args.batch // 2, 777)
         best, score, hist = tune_batch(dg, roots, rounds=args.rounds, log=log, budget_s=args.budget)
         name = f"rmat{args.scale}_batch"
-        meta = {"graph": f"rmat{args.scale} ef16 seed {args.scale}", "roots": args.batch,
+        meta = {"graph": f"rmat{args.scale} ef16 seed {seed}", "roots": args.batch,
                 "roots_kind": "half top-1% degree, half degree 1..4 (seed 777)", "candidates": len(hist),
                 "score": "aggregate GTEPS of one multi-source search"}
     else:
@@ -224,12 +233,9 @@ def main():
             dg.configure(**{k: v for k, v in start.items() if k not in ("reserved", "launch")})
         best, score, hist = tune(dg, roots, roots[:3], rounds=args.rounds, log=log, budget_s=args.budget)
         name = f"rmat{args.scale}"
-        meta = {"graph": f"rmat{args.scale} ef16 seed {args.scale}", "roots": args.roots,
+        meta = {"graph": f"rmat{args.scale} ef16 seed {seed}", "roots": args.roots,
                 "candidates": len(hist)}
-    if args.out:
-        global TUNED_DIR
-        TUNED_DIR = args.out
-    path = save(name, best, score, meta)
+    path = save(name, best, score, meta, args.out)
     print(json.dumps({"tuned": path, "score": score, "config": best}))
 
 

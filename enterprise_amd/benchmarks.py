@@ -97,6 +97,7 @@ def batch(args):
     plain_agg = plain_edges / plain_time / 1e9
     achieved = best["alg_bytes"] / (best["device_ms"] * 1e-3) / 1e9
     dg.free()
+    tr = traffic.figure(f"batch_rmat{scale}")
     return {
         "mode": "batch", "metric": "aggregate GTEPS, 512-source batched vs plain",
         "value": round(batch_gteps, 2), "unit": "GTEPS", "dtype": "int32", "data": "synthetic",
@@ -110,8 +111,11 @@ def batch(args):
         "rows_mismatching_plain": mism, "levels": best["levels"], "phases": best["phases"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 4), "alg_bytes": best["alg_bytes"],
-                     "traffic": traffic.figure(f"batch_rmat{scale}")},
+                     "traffic": tr, "traffic_over_alg": round(tr / best["alg_bytes"], 3) if tr else None},
         "runs_ms": [round(s["host_ms"], 3) for s in stats],
+        "fallback_rows": best["fallback_rows"],
+        # The graph and roots, for the CPU baseline that bench.py runs.
+        "_cpu_batch": {"graph": g, "roots": roots},
     }
 
 
@@ -124,9 +128,9 @@ def grid(args):
     on a search of ~8,000 levels: per-phase launches queued on a stream,
     per-phase launches replayed from a captured hipGraph, one persistent
     cooperative launch with grid barriers, and the host-synchronised loop of
-    the reference. Every root passes the device all-vertex check; two roots
-    are checked against the CPU oracle, which is also the CPU baseline (both
-    in bench.py)."""
+    the reference. Every root passes the device all-vertex check; one root per
+    launch form is checked against the CPU oracle, which is also the CPU
+    baseline (both in bench.py)."""
     import sys
 
     import numpy as np
@@ -144,8 +148,13 @@ def grid(args):
     roots = g.pick_roots(16, 1003)
     log(f"[grid] {side}x{side}: n={g.n} m={g.m} ({time.perf_counter() - t0:.1f}s)")
     dg = DeviceGraph(g)
-    forms = {"stream": (L.LAUNCH_STREAM, 16), "graph": (L.LAUNCH_GRAPH, 16),
-             "persistent": (L.LAUNCH_PERSISTENT, 4), "levels": (L.LAUNCH_LEVELS, 2)}
+    # Roots timed per form (the slower forms on fewer roots); --grid-roots
+    # scales every count down for a shorter run.
+    scale_roots = float(getattr(args, "grid_roots", 1.0) or 1.0)
+    forms = {name: (launch, max(1, int(round(nr * scale_roots))))
+             for name, (launch, nr) in {"stream": (L.LAUNCH_STREAM, 16), "graph": (L.LAUNCH_GRAPH, 16),
+                                         "persistent": (L.LAUNCH_PERSISTENT, 4),
+                                         "levels": (L.LAUNCH_LEVELS, 2)}.items()}
     only = [f for f in (getattr(args, "grid_forms", "") or "").split(",") if f]
     if only:
         forms = {k: v for k, v in forms.items() if k in only}
@@ -167,11 +176,15 @@ def grid(args):
                      "phases_per_level": round(float(np.mean(phases)) / float(np.mean(levels)), 3),
                      "levels_mean": round(float(np.mean(levels)), 1)}
         log(f"[grid] {name}: {res[name]}")
+    # Depths of one root per form (a different root each) and the device
+    # all-vertex check of every root; the CPU baseline and the parity check
+    # against the oracle run in bench.py on the depths handed back under
+    # "_cpu_check".
+    checks = []
+    for j, (name, (launch, _)) in enumerate(forms.items()):
+        dg.configure(launch=launch, graph_levels=96, **extra)
+        checks.append((int(roots[j]), dg.bfs(int(roots[j]))[0], name))
     dg.configure(launch=L.LAUNCH_STREAM)
-    # Depths of two roots and the device all-vertex check of every root; the
-    # CPU baseline and the parity check against the oracle run in bench.py on
-    # the depths handed back under "_cpu_check".
-    checks = [(int(r), dg.bfs(int(r))[0]) for r in roots[:2]]
     violations = 0
     for r in roots:
         dg.bfs_device(int(r))
@@ -235,7 +248,7 @@ def serve(args):
         for rate in rates:
             for lane in srv.lanes.values():
                 lane.stats.update({"batches": 0, "plain_batches": 0, "queries": 0, "reached_edges": 0,
-                                   "alg_bytes": 0, "busy_s": 0.0, "batch_sizes": []})
+                                   "alg_bytes": 0, "busy_s": 0.0, "batch_sizes": [], "fallback_rows": 0})
             t1 = time.perf_counter()
             res = run_load(srv, rate, dur, 1005 + rate, roots, {"large": 0.8, "small": 0.2})
             wall = time.perf_counter() - t1
@@ -257,6 +270,7 @@ def serve(args):
                    "throughput_qps": round(len(res) / wall, 1), "gteps": round(edges / wall / 1e9, 2),
                    "alg_GBps": round(alg / wall / 1e9, 1),
                    "mean_batch": round(float(np.mean(sizes)), 1) if sizes else 0,
+                   "fallback_rows": sum(l.stats["fallback_rows"] for l in srv.lanes.values()),
                    "checked": int(len(sample)), "mismatches": mism}
             log(f"[serve] {row}")
             per_rate.append(row)

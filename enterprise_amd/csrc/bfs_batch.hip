@@ -111,6 +111,7 @@ struct BArgs {
   int mode, alpha;
   int probe;
   int chunk_words;
+  int out16;  // out is 16-byte aligned and n % 4 == 0: rows in 16-byte stores
 };
 
 template <int W>
@@ -246,12 +247,16 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
     s.phase = BPH_OUTPUT;
     return s;
   }
+  for (int w = 0; w < 8; ++w) s.active[w] = w < W ? tot[C_ACTIVE + w] : 0ull;
   if (L + 1 > a.lcap) {
+    // Out of reserved levels: the output pass writes what levels 0..L found,
+    // and the sources still active (s.active) are left unfinished for the
+    // host to search one at a time.
     s.error |= BERR_LEVELS;
-    s.done = 1;
+    s.levels = L + 1;
+    s.phase = BPH_OUTPUT;
     return s;
   }
-  for (int w = 0; w < 8; ++w) s.active[w] = w < W ? tot[C_ACTIVE + w] : 0ull;
   const unsigned long long mf = tot[C_MF], munf = tot[C_MUNF];
   // Direction per 64-source word: a word whose new frontier is large is
   // pulled, a word that lags is pushed, so a pull can stop once the large
@@ -663,6 +668,12 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
   const int wid = threadIdx.x / kWave, lane = lane_id();
   unsigned char *d8 = sm.d8[wid];
   unsigned long long edges = 0, verts = 0;
+  // The reached counts leave out the sources a truncated search left
+  // unfinished (their rows are searched again one at a time).
+  const bool trunc = (sm.S.error & BERR_LEVELS) != 0;
+  Mask<W> cm;
+#pragma unroll
+  for (int w = 0; w < W; ++w) cm.w[w] = trunc ? ~sm.S.active[w] : ~0ull;
   for (long long wd = blockIdx.x; wd < a.nwords; wd += gridDim.x) {
     const long long v = wd * 64 + lane;
     const bool valid = v < a.n;
@@ -670,7 +681,7 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
       const Mask<W> sd = load_mask<W>(a.seen, v);
       int pc = 0;
 #pragma unroll
-      for (int w = 0; w < W; ++w) pc += __popcll(sd.w[w]);
+      for (int w = 0; w < W; ++w) pc += __popcll(sd.w[w] & cm.w[w]);
       verts += (unsigned long long)pc;
       edges += (unsigned long long)pc * (unsigned long long)(a.off[v + 1] - a.off[v]);
     }
@@ -690,10 +701,10 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
       }
       wave_sync();
       const int nb = min(64, a.k - 64 * g);
-      if ((a.n & 3) == 0) {
+      if (a.out16) {
         // 16-byte streaming stores: lane l writes vertices 4(l%16) .. +3 of
-        // the segment for source 4j + l/16, j = 0..15 (rows are 16-byte
-        // aligned when n is a multiple of 4).
+        // the segment for source 4j + l/16, j = 0..15 (every row starts on a
+        // 16-byte boundary: out is 16-byte aligned and n a multiple of 4).
         const int q = lane & 15, sb = lane >> 4;
         const long long v0 = wd * 64 + 4 * q;
         if (v0 < a.n) {
@@ -762,14 +773,20 @@ __global__ __launch_bounds__(BS) void k_blevel_bu(BArgs a, long long i) {
 // completion, or when a barrier wait exceeds ~4 s in any block (the search is
 // then marked failed). Exit decisions are taken from LDS after a block
 // barrier, so a block leaves the loop as a whole.
+// Wavefronts per SIMD the persistent kernel is compiled for (its register
+// budget guarantees at least this many); the cooperative grid is sized from
+// the same figure, so every block is resident.
+constexpr int bpersist_waves_per_simd(int W) { return W >= 8 ? 3 : 4; }
+
 template <int BS, int W>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W >= 8 ? 3 : 4))) void k_bpersist(BArgs a) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(bpersist_waves_per_simd(W)))) void k_bpersist(
+    BArgs a) {
   __shared__ BSmem<BS> sm;
   __shared__ int stop;
   for (long long i = 0;; ++i) {
     begin_blaunch<BS, W>(a, sm, i, true);
     const BState &S = sm.S;
-    if (S.done || S.error) break;
+    if (S.done || (S.error & BERR_TIMEOUT)) break;
     const int slot = int(i % 3);
     switch (S.phase) {
       case BPH_TD: btd_phase<BS, W>(a, sm, S.level); break;
@@ -806,7 +823,7 @@ __global__ void k_binit_zero(BArgs a) {
   }
   constexpr long long kC = (long long)kBStripes * kBStripeU64;
   for (long long i = gt; i < 3 * kC; i += nt) a.ctl->cnt[i / kC].v[i % kC] = 0;
-  for (long long i = gt; i < kSyncStripes * 32; i += nt) a.ctl->sync.arrive[i] = 0;
+  for (long long i = gt; i < kSyncStripes * kSyncStride; i += nt) a.ctl->sync.arrive[i] = 0;
   if (gt == 0) {
     a.ctl->sync.arrive_all = 0;
     a.ctl->sync.gen = 0;
@@ -882,6 +899,8 @@ struct BatchWorkspace {
   BControl *h_ctl = nullptr;
   size_t bytes = 0;
   int expect_launches = 16;
+  int *rows = nullptr;     // device rows of ent_bfs_batch (grown on demand)
+  size_t rows_bytes = 0;
 };
 
 }  // namespace ent
@@ -987,6 +1006,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   a.alpha = c.batch_alpha > 0 ? c.batch_alpha : 4;
   a.probe = c.bu_probe;
   a.chunk_words = c.bu_chunk_words;
+  a.out16 = ((reinterpret_cast<uintptr_t>(out) & 15) == 0 && (g->n & 3) == 0) ? 1 : 0;
   // Level-0 counts on the host: distinct (vertex, word) pairs of the roots.
   std::vector<std::pair<int32_t, int>> vw;
   std::vector<int32_t> vs;
@@ -1020,12 +1040,14 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   }
   const BKernels kl = blevel_for(W);
   if (c.launch == ENT_LAUNCH_PERSISTENT) {
-    // Every block must be resident: at most 16 wavefronts and 8 workgroups
-    // per CU (the kernel is compiled for at least 3 or 4 wavefronts per
-    // SIMD), and within the occupancy the runtime reports.
+    // Every block must be resident: no more wavefronts per CU than the
+    // kernel's register budget guarantees (4 SIMDs x bpersist_waves_per_simd
+    // wavefronts: 3 blocks of 256 at W = 8, 4 below), at most 8 workgroups,
+    // and within the occupancy the runtime reports.
     int occ = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kl.persist, kBatchBS, 0));
-    int per_cu = std::min(std::min(occ, 16 / (kBatchBS / 64)), 8);
+    const int wave_cap = 4 * bpersist_waves_per_simd(W) / (kBatchBS / kWave);
+    int per_cu = std::min(std::min(occ, wave_cap), 8);
     if (c.blocks_per_cu > 0) per_cu = std::min(per_cu, c.blocks_per_cu);
     if (per_cu < 1) return set_error(ENT_EDEVICE, "persistent multi-source kernel cannot be resident");
     int pgrid = g->cus * per_cu;
@@ -1075,11 +1097,77 @@ void batch_release(ent_graph *g) {
   dev_free(g, b->jf[1], 8 * size_t(g->nwords));
   dev_free(g, b->roots, 4 * ENT_BATCH_MAX_SOURCES);
   dev_free(g, b->ctl, sizeof(BControl));
+  dev_free(g, b->rows, b->rows_bytes);
   if (b->h_ctl) ignore(hipHostFree(b->h_ctl));
   delete b;
   g->batch = nullptr;
 }
 }  // namespace ent
+
+namespace {
+
+// Multi-source search with g->mu held. A search that runs out of reserved
+// levels writes the rows its levels settled, and the sources it left
+// unfinished are searched again one at a time into their rows (counted in
+// st->fallback_rows), so every row is complete whatever the depth.
+int batch_locked(ent_graph *g, const int32_t *roots, int k, int32_t *depths_dev, ent_batch_stats *st,
+                 std::chrono::steady_clock::time_point t0) {
+  HIP_TRY(hipSetDevice(g->device));
+  int rc = reserve(g, k, g->batch ? g->batch->lcap : kDefaultLevels);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(g->ev0, g->stream));
+  BState fin{};
+  rc = run_batch(g, roots, k, depths_dev, &fin);
+  if (rc) {
+    ignore(hipStreamSynchronize(g->stream));
+    return rc;
+  }
+  if (fin.error & BERR_TIMEOUT) {
+    ignore(hipStreamSynchronize(g->stream));
+    return set_error(ENT_EDEVICE, "persistent multi-source search: grid barrier wait timed out");
+  }
+  if (fin.error & ~BERR_LEVELS) {
+    ignore(hipStreamSynchronize(g->stream));
+    return set_error(ENT_EINTERNAL, "device reported error flags 0x%x", fin.error);
+  }
+  unsigned long long edges = fin.total_edges, verts = fin.total_vertices, alg = fin.alg_bytes;
+  int fallback = 0;
+  if (fin.error & BERR_LEVELS) {
+    for (int s = 0; s < k; ++s) {
+      if (!((fin.active[s >> 6] >> (s & 63)) & 1ull)) continue;
+      ent_stats one;
+      rc = search_locked(g, roots[s], depths_dev + size_t(s) * size_t(g->n), &one, t0);
+      if (rc) return rc;
+      edges += (unsigned long long)one.reached_edges;
+      verts += (unsigned long long)one.reached_vertices;
+      alg += (unsigned long long)one.alg_bytes;
+      fin.levels = std::max(fin.levels, one.levels);
+      ++fallback;
+    }
+  }
+  HIP_TRY(hipEventRecord(g->ev1, g->stream));
+  HIP_TRY(hipEventSynchronize(g->ev1));
+  const auto t1 = std::chrono::steady_clock::now();
+  if (st) {
+    std::memset(st, 0, sizeof *st);
+    st->k = k;
+    st->words = g->batch->words;
+    st->levels = fin.levels;
+    st->phases = fin.phases;
+    st->reached_edges = int64_t(edges);
+    st->reached_vertices = int64_t(verts);
+    st->alg_bytes = int64_t(alg);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g->ev0, g->ev1) == hipSuccess) st->device_ms = ms;
+    st->host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    st->bu_level_mask = fin.bu_mask;
+    st->td_level_mask = fin.td_mask;
+    st->fallback_rows = fallback;
+  }
+  return ENT_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1113,6 +1201,17 @@ int ent_bfs_batch_device(ent_graph *g, const int32_t *roots, int32_t k, int32_t 
   clear_error();
   if (!g || !roots || !depths_dev) return set_error(ENT_EINVAL, "NULL argument");
   if (k < 1 || k > ENT_BATCH_MAX_SOURCES) return set_error(ENT_EINVAL, "k=%d outside [1, 512]", k);
+  if (reinterpret_cast<uintptr_t>(depths_dev) & 3) return set_error(ENT_EINVAL, "depths_dev not 4-byte aligned");
+  for (int s = 0; s < k; ++s)
+    if (roots[s] < 0 || roots[s] >= g->n) return set_error(ENT_EINVAL, "root %d = %d out of range", s, roots[s]);
+  std::lock_guard<std::mutex> lk(g->mu);
+  return batch_locked(g, roots, k, depths_dev, st, std::chrono::steady_clock::now());
+}
+
+int ent_bfs_batch(ent_graph *g, const int32_t *roots, int32_t k, int32_t *depths_host, ent_batch_stats *st) {
+  clear_error();
+  if (!g || !roots || !depths_host) return set_error(ENT_EINVAL, "NULL argument");
+  if (k < 1 || k > ENT_BATCH_MAX_SOURCES) return set_error(ENT_EINVAL, "k=%d outside [1, 512]", k);
   for (int s = 0; s < k; ++s)
     if (roots[s] < 0 || roots[s] >= g->n) return set_error(ENT_EINVAL, "root %d = %d out of range", s, roots[s]);
   std::lock_guard<std::mutex> lk(g->mu);
@@ -1120,50 +1219,19 @@ int ent_bfs_batch_device(ent_graph *g, const int32_t *roots, int32_t k, int32_t 
   HIP_TRY(hipSetDevice(g->device));
   int rc = reserve(g, k, g->batch ? g->batch->lcap : kDefaultLevels);
   if (rc) return rc;
-  HIP_TRY(hipEventRecord(g->ev0, g->stream));
-  BState fin{};
-  rc = run_batch(g, roots, k, depths_dev, &fin);
-  if (rc) {
-    ignore(hipStreamSynchronize(g->stream));
-    return rc;
+  // Device rows kept with the workspace and reused by later calls.
+  BatchWorkspace *b = g->batch;
+  const size_t bytes = sizeof(int32_t) * size_t(k) * size_t(g->n);
+  if (b->rows_bytes < bytes) {
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    dev_free(g, b->rows, b->rows_bytes);
+    b->rows = nullptr;
+    b->rows_bytes = 0;
+    if ((rc = dev_alloc(g, reinterpret_cast<void **>(&b->rows), bytes))) return rc;
+    b->rows_bytes = bytes;
   }
-  HIP_TRY(hipEventRecord(g->ev1, g->stream));
-  HIP_TRY(hipEventSynchronize(g->ev1));
-  const auto t1 = std::chrono::steady_clock::now();
-  if (fin.error & BERR_LEVELS)
-    return set_error(ENT_ENOMEM, "batch needs more than %d joint levels; reserve more with ent_batch_reserve",
-                     g->batch->lcap);
-  if (fin.error & BERR_TIMEOUT)
-    return set_error(ENT_EDEVICE, "persistent multi-source search: grid barrier wait timed out");
-  if (fin.error) return set_error(ENT_EINTERNAL, "device reported error flags 0x%x", fin.error);
-  if (st) {
-    std::memset(st, 0, sizeof *st);
-    st->k = k;
-    st->words = g->batch->words;
-    st->levels = fin.levels;
-    st->phases = fin.phases;
-    st->reached_edges = int64_t(fin.total_edges);
-    st->reached_vertices = int64_t(fin.total_vertices);
-    st->alg_bytes = int64_t(fin.alg_bytes);
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, g->ev0, g->ev1) == hipSuccess) st->device_ms = ms;
-    st->host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    st->bu_level_mask = fin.bu_mask;
-    st->td_level_mask = fin.td_mask;
-  }
-  return ENT_OK;
-}
-
-int ent_bfs_batch(ent_graph *g, const int32_t *roots, int32_t k, int32_t *depths_host, ent_batch_stats *st) {
-  clear_error();
-  if (!g || !depths_host) return set_error(ENT_EINVAL, "NULL argument");
-  if (k < 1 || k > ENT_BATCH_MAX_SOURCES) return set_error(ENT_EINVAL, "k=%d outside [1, 512]", k);
-  void *dev = nullptr;
-  int rc = ent_device_alloc(g, int64_t(k) * g->n * 4, &dev);
-  if (rc) return rc;
-  rc = ent_bfs_batch_device(g, roots, k, static_cast<int32_t *>(dev), st);
-  if (!rc) rc = ent_copy_to_host(g, static_cast<int32_t *>(dev), depths_host, int64_t(k) * g->n);
-  ent_device_free(g, dev);
+  rc = batch_locked(g, roots, k, b->rows, st, t0);
+  if (!rc) rc = copy_to_host(g, depths_host, b->rows, bytes);
   return rc;
 }
 
@@ -1192,24 +1260,25 @@ int ent_device_free(ent_graph *g, void *p) {
 
 int ent_device_row_hash(ent_graph *g, const int32_t *dev, int64_t rows, int64_t cols, uint64_t *hashes_host) {
   clear_error();
-  if (!g || !dev || !hashes_host || rows < 1 || cols < 1 || rows > 65535)
-    return set_error(ENT_EINVAL, "bad argument");
+  if (!g || !dev || !hashes_host || rows < 1 || cols < 1) return set_error(ENT_EINVAL, "bad argument");
   std::lock_guard<std::mutex> lk(g->mu);
   HIP_TRY(hipSetDevice(g->device));
-  unsigned long long *d = nullptr;
-  HIP_TRY(hipMalloc(&d, sizeof(unsigned long long) * size_t(rows)));
+  // The handle's own checksum buffer, kHashRows rows per launch: no
+  // allocation, so calls on other handles' streams keep running meanwhile.
+  unsigned long long *d = g->hashbuf;
   long long chunk = 1 << 16;
   const long long nchunks = (cols + chunk - 1) / chunk;
-  void *args[] = {const_cast<int32_t **>(&dev), &cols, &chunk, &d};
-  hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long) * size_t(rows), g->stream);
-  if (e == hipSuccess)
-    e = hipLaunchKernel(reinterpret_cast<const void *>(&k_row_hash), dim3(unsigned(nchunks), unsigned(rows)), dim3(256),
-                        args, 0, g->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(hashes_host, d, sizeof(unsigned long long) * size_t(rows), hipMemcpyDeviceToHost, g->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
-  ignore(hipFree(d));
-  if (e != hipSuccess) return set_error(ENT_EDEVICE, "row hash: %s", hipGetErrorString(e));
+  for (int64_t r0 = 0; r0 < rows; r0 += kHashRows) {
+    const int64_t nr = std::min<int64_t>(kHashRows, rows - r0);
+    const int32_t *base = dev + r0 * cols;
+    void *args[] = {const_cast<int32_t **>(&base), &cols, &chunk, &d};
+    HIP_TRY(hipMemsetAsync(d, 0, sizeof(unsigned long long) * size_t(nr), g->stream));
+    HIP_TRY(hipLaunchKernel(reinterpret_cast<const void *>(&k_row_hash), dim3(unsigned(nchunks), unsigned(nr)),
+                            dim3(256), args, 0, g->stream));
+    HIP_TRY(hipMemcpyAsync(hashes_host + r0, d, sizeof(unsigned long long) * size_t(nr), hipMemcpyDeviceToHost,
+                           g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+  }
   return ENT_OK;
 }
 

@@ -909,19 +909,23 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
     if (lane == 0) {
       const int prev_level = sm.S.level;
       next_state(a, sm.S, tot, q_fit);
-      if (COUNT && blockIdx.x == 0 && !sm.S.done && i < kTraceMax) {
-        // The counts of launch i - 1 are the new level's frontier when the
+      // Record index: the launch's number within the search (phases - 1),
+      // not the kernel argument i, which restarts at 0 on every hipGraph
+      // replay.
+      const long long li = (long long)sm.S.phases - 1;
+      if (COUNT && blockIdx.x == 0 && !sm.S.done && li < kTraceMax) {
+        // The counts of launch li - 1 are the new level's frontier when the
         // level advanced; a launch that continues a level (inspection,
         // the phase after a rebuild) repeats the previous record's counts.
         LaunchRec r;
-        r.launch = int(i);
+        r.launch = int(li);
         r.level = sm.S.level;
         r.phase = sm.S.phase;
         r.pad = 0;
-        const bool same = i > 0 && sm.S.level == prev_level;
-        r.nf = same ? ld_agent(&c->trace[i - 1].nf) : tot[K_NF];
-        r.mf = same ? ld_agent(&c->trace[i - 1].mf) : tot[K_MF];
-        c->trace[i] = r;
+        const bool same = li > 0 && sm.S.level == prev_level;
+        r.nf = same ? ld_agent(&c->trace[li - 1].nf) : tot[K_NF];
+        r.mf = same ? ld_agent(&c->trace[li - 1].mf) : tot[K_MF];
+        c->trace[li] = r;
       }
       sm.nf = sm.mf = 0;
       sm.qcount = 0;
@@ -993,7 +997,7 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu, int
   }
   if (gt == 0) a.qbuf[0][0] = root;
   for (long long s = gt; s <= a.hub_mask; s += nt) a.hub_tag[s] = a.hub_slot[s] == root ? 0 : -1;
-  for (long long i = gt; i < kSyncStripes * 32; i += nt) c->sync.arrive[i] = 0;
+  for (long long i = gt; i < kSyncStripes * kSyncStride; i += nt) c->sync.arrive[i] = 0;
   if (gt == 0) {
     State S{};
     S.level = -1;
@@ -1113,6 +1117,7 @@ void release(ent_graph *g) {
   ignore(hipFree(g->piece_k));
   ignore(hipFree(g->depth));
   ignore(hipFree(g->aux));
+  ignore(hipFree(g->hashbuf));
   ignore(hipFree(g->parent));
   ignore(hipFree(g->visited));
   ignore(hipFree(g->front[0]));
@@ -1555,7 +1560,8 @@ int alloc_workspace(ent_graph *g) {
       (rc = dev_alloc(g, reinterpret_cast<void **>(&g->qbuf[0]), sizeof(int) * size_t(kStripes) * size_t(g->qcap_s))) ||
       (rc = dev_alloc(g, reinterpret_cast<void **>(&g->qbuf[1]), sizeof(int) * size_t(kStripes) * size_t(g->qcap_s))) ||
       (rc = dev_alloc(g, reinterpret_cast<void **>(&g->ctl), sizeof(Control))) ||
-      (rc = dev_alloc(g, &g->aux, 256)))
+      (rc = dev_alloc(g, &g->aux, 256)) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&g->hashbuf), sizeof(unsigned long long) * kHashRows)))
     return rc;
   HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g->h_ctl), sizeof(Control), hipHostMallocDefault));
   std::memset(g->h_ctl, 0, sizeof(Control));
@@ -1734,6 +1740,103 @@ int upload_graph(const int64_t *beg_pos, const int32_t *adj, bool adj_on_device,
   return ENT_OK;
 }
 
+
+// Device-to-host copy on the handle's stream. Pinned destinations (and small
+// copies) take one DMA; pageable ones go through two pinned staging buffers:
+// the DMA of chunk c overlaps the multi-threaded host copy of chunk c - 1
+// (a DMA straight into pageable memory runs at a few GB/s).
+int copy_to_host(ent_graph *g, void *dst, const void *src, size_t bytes) {
+  constexpr size_t kChunk = size_t(8) << 20;
+  hipPointerAttribute_t attr;
+  bool pinned = false;
+  if (hipPointerGetAttributes(&attr, dst) == hipSuccess) pinned = attr.type == hipMemoryTypeHost;
+  else ignore(hipGetLastError());  // pageable memory is unknown to the runtime
+  if (pinned || bytes <= kChunk) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return ENT_OK;
+  }
+  for (int b = 0; b < 2; ++b) {
+    if (!g->stage[b]) HIP_TRY(hipHostMalloc(&g->stage[b], kChunk, hipHostMallocDefault));
+    if (!g->stage_ev[b]) HIP_TRY(hipEventCreateWithFlags(&g->stage_ev[b], hipEventDisableTiming));
+  }
+  const size_t nchunks = (bytes + kChunk - 1) / kChunk;
+  auto chunk_bytes = [&](size_t c) { return std::min(kChunk, bytes - c * kChunk); };
+  auto drain = [&](size_t c) -> int {  // host copy of chunk c out of its staging buffer
+    HIP_TRY(hipEventSynchronize(g->stage_ev[c & 1]));
+    const size_t cb = chunk_bytes(c);
+    char *d = static_cast<char *>(dst) + c * kChunk;
+    const char *sb = static_cast<const char *>(g->stage[c & 1]);
+    parallel_for(int64_t(cb), int64_t(1) << 20, [&](int64_t b0, int64_t e0) {
+      std::memcpy(d + b0, sb + b0, size_t(e0 - b0));
+    });
+    return ENT_OK;
+  };
+  // Buffer c & 1 is free when chunk c is issued: chunk c - 2 was drained in
+  // the previous iteration.
+  for (size_t c = 0; c < nchunks; ++c) {
+    HIP_TRY(hipMemcpyAsync(g->stage[c & 1], static_cast<const char *>(src) + c * kChunk, chunk_bytes(c),
+                           hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipEventRecord(g->stage_ev[c & 1], g->stream));
+    if (c >= 1) {
+      const int rc = drain(c - 1);
+      if (rc) return rc;
+    }
+  }
+  return drain(nchunks - 1);
+}
+
+// One single-source search from root into depth (device memory), with the
+// handle's lock held and ev0 already recorded; fills *st (may be NULL). t0 is
+// the start of the caller's host window.
+int search_locked(ent_graph *g, int root, int *depth, ent_stats *st, std::chrono::steady_clock::time_point t0) {
+  State c{};
+  g->trace_launches = 0;
+  int rc = run_search(g, root, depth, &c);
+  if (rc) {
+    ignore(hipStreamSynchronize(g->stream));
+    return rc;
+  }
+  // The search is finished; launches queued behind the one that found it
+  // finished may still be on the stream (they exit at once).
+  const auto t1 = std::chrono::steady_clock::now();
+  if (c.error)
+    return set_error(ENT_EINTERNAL, "device reported error flags 0x%x (2 level guard, 4 barrier timeout)", c.error);
+  if (!c.done) return set_error(ENT_EINTERNAL, "search ended without completion");
+  // The last counted launch finds the frontier empty and records nothing.
+  if (g->cfg.count_examined) g->trace_launches = c.phases - 1;
+  if (st) {
+    std::memset(st, 0, sizeof *st);
+    st->levels = c.levels;
+    st->td_levels = c.td_levels;
+    st->bu_levels = c.bu_levels;
+    st->phases = c.phases;
+    st->reached_vertices = int64_t(c.total_nf);
+    st->reached_edges = int64_t(c.total_mf);
+    st->alg_bytes = 8 * st->reached_edges + 12 * st->reached_vertices + 4 * g->n;
+    // Device clock (100 MHz) from k_init to the launch that found the search
+    // finished; the HIP-event time of the same search is ent_last_device_ms.
+    if (c.t_end > c.t_begin) st->device_ms = double(c.t_end - c.t_begin) * 1e-5;
+    st->host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    st->bu_level_mask = c.bu_mask;
+    st->grid_blocks = g->last_grid;
+    st->block_size = g->cfg.block_size;
+    st->examined_bytes = -1;
+    st->examined_edges = -1;
+    if (g->cfg.count_examined) {
+      unsigned long long ex[kStripes * 4];
+      HIP_TRY(hipStreamSynchronize(g->stream));
+      HIP_TRY(hipMemcpy(ex, g->ctl->exam, sizeof ex, hipMemcpyDeviceToHost));
+      unsigned long long t[3] = {0, 0, 0};
+      for (int k = 0; k < kStripes; ++k)
+        for (int j = 0; j < 3; ++j) t[j] += ex[k * 4 + j];
+      st->examined_edges = int64_t(t[0]);
+      st->examined_bytes = int64_t(8 * t[0] + 8 * t[1] + t[2]);
+    }
+  }
+  return ENT_OK;
+}
+
 }  // namespace ent
 
 extern "C" {
@@ -1824,53 +1927,8 @@ int ent_bfs_device(ent_graph *g, int32_t root, int32_t *depth_dev, ent_stats *st
   std::lock_guard<std::mutex> lk(g->mu);
   const auto t0 = std::chrono::steady_clock::now();
   HIP_TRY(hipSetDevice(g->device));
-  int *depth = depth_dev ? depth_dev : g->depth;
   HIP_TRY(hipEventRecord(g->ev0, g->stream));
-  State c{};
-  g->trace_launches = 0;
-  int rc = run_search(g, root, depth, &c);
-  if (rc) {
-    ignore(hipStreamSynchronize(g->stream));
-    return rc;
-  }
-  // The search is finished; launches queued behind the one that found it
-  // finished may still be on the stream (they exit at once).
-  const auto t1 = std::chrono::steady_clock::now();
-  if (c.error)
-    return set_error(ENT_EINTERNAL, "device reported error flags 0x%x (2 level guard, 4 barrier timeout)", c.error);
-  if (!c.done) return set_error(ENT_EINTERNAL, "search ended without completion");
-  // The last counted launch finds the frontier empty and records nothing.
-  if (g->cfg.count_examined) g->trace_launches = c.phases - 1;
-  if (st) {
-    std::memset(st, 0, sizeof *st);
-    st->levels = c.levels;
-    st->td_levels = c.td_levels;
-    st->bu_levels = c.bu_levels;
-    st->phases = c.phases;
-    st->reached_vertices = int64_t(c.total_nf);
-    st->reached_edges = int64_t(c.total_mf);
-    st->alg_bytes = 8 * st->reached_edges + 12 * st->reached_vertices + 4 * g->n;
-    // Device clock (100 MHz) from k_init to the launch that found the search
-    // finished; the HIP-event time of the same search is ent_last_device_ms.
-    if (c.t_end > c.t_begin) st->device_ms = double(c.t_end - c.t_begin) * 1e-5;
-    st->host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    st->bu_level_mask = c.bu_mask;
-    st->grid_blocks = g->last_grid;
-    st->block_size = g->cfg.block_size;
-    st->examined_bytes = -1;
-    st->examined_edges = -1;
-    if (g->cfg.count_examined) {
-      unsigned long long ex[kStripes * 4];
-      HIP_TRY(hipStreamSynchronize(g->stream));
-      HIP_TRY(hipMemcpy(ex, g->ctl->exam, sizeof ex, hipMemcpyDeviceToHost));
-      unsigned long long t[3] = {0, 0, 0};
-      for (int k = 0; k < kStripes; ++k)
-        for (int j = 0; j < 3; ++j) t[j] += ex[k * 4 + j];
-      st->examined_edges = int64_t(t[0]);
-      st->examined_bytes = int64_t(8 * t[0] + 8 * t[1] + t[2]);
-    }
-  }
-  return ENT_OK;
+  return ent::search_locked(g, root, depth_dev ? depth_dev : g->depth, st, t0);
 }
 
 int ent_last_device_ms(ent_graph *g, double *ms) {
@@ -1889,51 +1947,6 @@ int ent_bfs(ent_graph *g, int32_t root, int32_t *depth_host, ent_stats *st) {
   int rc = ent_bfs_device(g, root, nullptr, st);
   if (rc || !depth_host) return rc;
   return ent_copy_to_host(g, g->depth, depth_host, g->n);
-}
-
-// Device-to-host copy on the handle's stream. Pinned destinations (and small
-// copies) take one DMA; pageable ones go through two pinned staging buffers:
-// the DMA of chunk c overlaps the multi-threaded host copy of chunk c - 1
-// (a DMA straight into pageable memory runs at a few GB/s).
-int copy_to_host(ent_graph *g, void *dst, const void *src, size_t bytes) {
-  constexpr size_t kChunk = size_t(8) << 20;
-  hipPointerAttribute_t attr;
-  bool pinned = false;
-  if (hipPointerGetAttributes(&attr, dst) == hipSuccess) pinned = attr.type == hipMemoryTypeHost;
-  else ignore(hipGetLastError());  // pageable memory is unknown to the runtime
-  if (pinned || bytes <= kChunk) {
-    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
-    return ENT_OK;
-  }
-  for (int b = 0; b < 2; ++b) {
-    if (!g->stage[b]) HIP_TRY(hipHostMalloc(&g->stage[b], kChunk, hipHostMallocDefault));
-    if (!g->stage_ev[b]) HIP_TRY(hipEventCreateWithFlags(&g->stage_ev[b], hipEventDisableTiming));
-  }
-  const size_t nchunks = (bytes + kChunk - 1) / kChunk;
-  auto chunk_bytes = [&](size_t c) { return std::min(kChunk, bytes - c * kChunk); };
-  auto drain = [&](size_t c) -> int {  // host copy of chunk c out of its staging buffer
-    HIP_TRY(hipEventSynchronize(g->stage_ev[c & 1]));
-    const size_t cb = chunk_bytes(c);
-    char *d = static_cast<char *>(dst) + c * kChunk;
-    const char *sb = static_cast<const char *>(g->stage[c & 1]);
-    parallel_for(int64_t(cb), int64_t(1) << 20, [&](int64_t b0, int64_t e0) {
-      std::memcpy(d + b0, sb + b0, size_t(e0 - b0));
-    });
-    return ENT_OK;
-  };
-  // Buffer c & 1 is free when chunk c is issued: chunk c - 2 was drained in
-  // the previous iteration.
-  for (size_t c = 0; c < nchunks; ++c) {
-    HIP_TRY(hipMemcpyAsync(g->stage[c & 1], static_cast<const char *>(src) + c * kChunk, chunk_bytes(c),
-                           hipMemcpyDeviceToHost, g->stream));
-    HIP_TRY(hipEventRecord(g->stage_ev[c & 1], g->stream));
-    if (c >= 1) {
-      const int rc = drain(c - 1);
-      if (rc) return rc;
-    }
-  }
-  return drain(nchunks - 1);
 }
 
 int ent_copy_to_host(ent_graph *g, const int32_t *src_dev, int32_t *dst_host, int64_t count) {

@@ -97,31 +97,38 @@ __device__ __forceinline__ uint64_t rt_clock() { return wall_clock64(); }
 
 // Grid barrier state for cooperative (all blocks resident) launches: 64
 // striped arrival counters one 128-byte line apart, a global arrival counter
-// and a generation number. All zero before the first use; every use leaves
-// the counters at zero again.
+// and a generation number. All zero before the first use of a launch. The
+// counters only grow: barrier b takes arrivals b * quota .. (b + 1) * quota - 1
+// of a stripe (no block arrives at barrier b + 1 before every block has
+// arrived at b), so no counter is ever reset and no reset has to be ordered
+// before the next barrier's arrivals.
 constexpr int kSyncStripes = 64;
+constexpr int kSyncStride = 16;  // u64 words between stripes
 struct GridSync {
-  unsigned int arrive_all;
+  unsigned long long arrive_all;
   unsigned int gen;
-  unsigned int pad[30];
-  unsigned int arrive[kSyncStripes * 32];
+  unsigned int pad[29];
+  unsigned long long arrive[kSyncStripes * kSyncStride];
 };
 
-// Two-level arrival: blocks count on their stripe, the last of each stripe on
-// arrive_all; returns true for the single last block. Thread 0 only.
+// Two-level arrival: blocks count on their stripe, the last of each stripe
+// (per barrier) on arrive_all; returns true for the single last block.
+// Thread 0 only, after the block's release fence. The last arriver of a
+// stripe acquires the stripe's arrivals before it counts on arrive_all with a
+// release-acquire increment, so the block that returns true has acquired
+// every block's release and publishes it with its generation store.
 __device__ __forceinline__ bool grid_arrive_last(GridSync *c) {
-  const unsigned G = gridDim.x;
-  const unsigned s = blockIdx.x % kSyncStripes;
-  const unsigned quota = (G - s + kSyncStripes - 1) / kSyncStripes;
-  const unsigned nstripes = G < kSyncStripes ? G : kSyncStripes;
-  unsigned *sc = &c->arrive[s * 32];
-  const unsigned t = __hip_atomic_fetch_add(sc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (t != quota - 1) return false;
-  st_agent(sc, 0u);
-  const unsigned t2 = __hip_atomic_fetch_add(&c->arrive_all, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (t2 != nstripes - 1) return false;
-  st_agent(&c->arrive_all, 0u);
-  return true;
+  const unsigned long long G = gridDim.x;
+  const unsigned long long s = blockIdx.x % kSyncStripes;
+  const unsigned long long quota = (G - s + kSyncStripes - 1) / kSyncStripes;
+  const unsigned long long nstripes = G < kSyncStripes ? G : kSyncStripes;
+  unsigned long long *sc = &c->arrive[s * kSyncStride];
+  const unsigned long long t = __hip_atomic_fetch_add(sc, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if ((t + 1) % quota != 0) return false;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const unsigned long long t2 =
+      __hip_atomic_fetch_add(&c->arrive_all, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  return (t2 + 1) % nstripes == 0;
 }
 
 // Grid barrier. Every wavefront first waits for its own stores to complete

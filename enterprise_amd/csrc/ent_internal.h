@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <memory>
 #include <mutex>
@@ -65,6 +66,7 @@ struct ent_graph {
   int lists_hub = -1, lists_hub_tw = -1;
   int *depth = nullptr;
   void *aux = nullptr;     // 256 bytes of device scratch for host-side calls (counters)
+  unsigned long long *hashbuf = nullptr;  // kHashRows row checksums (ent_device_row_hash)
   int *parent = nullptr;   // parent array of ent_bfs_parents (allocated on first use)
   int trace_launches = 0;  // launches recorded by the last counting search
   void *stage[2] = {nullptr, nullptr};  // pinned staging for copies to pageable host memory
@@ -103,6 +105,9 @@ struct ent_graph {
 
 namespace ent {
 
+// Rows one ent_device_row_hash launch sums (its per-handle buffer's length).
+constexpr int kHashRows = 512;
+
 inline int dev_alloc(ent_graph *g, void **p, size_t bytes) {
   if (bytes == 0) bytes = 8;
   hipError_t e = hipMalloc(p, bytes);
@@ -123,5 +128,15 @@ inline void dev_free(ent_graph *g, void *p, size_t bytes) {
 
 // Frees the batched workspace (defined in bfs_batch.hip).
 void batch_release(ent_graph *g);
+
+// Device-to-host copy on g's stream, returning when it is complete (pinned
+// or small destinations in one DMA, pageable ones through pinned staging).
+// The caller holds g->mu.
+int copy_to_host(ent_graph *g, void *dst, const void *src, size_t bytes);
+
+// One single-source search from root into the device array depth, with
+// g->mu held and g->ev0 already recorded on g's stream; fills *st (may be
+// NULL). t0 starts the host window of st->host_ms (bfs_plain.hip).
+int search_locked(ent_graph *g, int root, int *depth, ent_stats *st, std::chrono::steady_clock::time_point t0);
 
 }  // namespace ent

@@ -10,9 +10,12 @@
  *
  * Memory: besides the graph, a workspace of (max_levels + 3) * n * W * 8
  * bytes holds the per-level frontier masks (levels are replayed at the end
- * to write every output row in full, coalesced lines). A batch whose search
- * needs more levels than reserved fails with ENT_ENOMEM and leaves the handle
- * usable; reserve more levels and retry.
+ * to write every output row in full, coalesced lines). A search that needs
+ * more joint levels than reserved still returns every row complete: the
+ * rows of sources that finished within the reserved levels come from the
+ * multi-source search, and each source still unfinished is searched again on
+ * its own (the single-source path, ent_bfs_device) into its row; their
+ * number is ent_batch_stats.fallback_rows.
  */
 #ifndef ENT_BATCH_H
 #define ENT_BATCH_H
@@ -44,6 +47,9 @@ typedef struct ent_batch_stats {
   uint64_t td_level_mask;   /* bit l: level l pushed (top-down) some of its
                                words; both bits set: the level split its words
                                by frontier size */
+  int32_t fallback_rows;    /* rows searched again on the single-source path
+                               because the search ran out of reserved levels */
+  int32_t pad;
 } ent_batch_stats;
 
 /* Reserves the workspace for up to k sources and max_levels joint levels
@@ -58,12 +64,14 @@ int64_t ent_batch_workspace_bytes(const ent_graph *g, int32_t k,
                                   int32_t max_levels);
 
 /* Multi-source BFS from roots[0..k-1] (1 <= k <= 512, roots may repeat),
- * writing k * n int32 depths to device memory depths_dev. */
+ * writing k * n int32 depths to device memory depths_dev (4-byte aligned;
+ * rows go out in 16-byte stores when it is 16-byte aligned and n % 4 == 0). */
 int ent_bfs_batch_device(ent_graph *g, const int32_t *roots, int32_t k,
                          int32_t *depths_dev, ent_batch_stats *stats);
 
 /* Same, copying the k * n depths to host memory depths_host afterwards
- * (outside the timed search). */
+ * (outside the timed search). The device rows are kept with the handle's
+ * multi-source workspace and reused by later calls. */
 int ent_bfs_batch(ent_graph *g, const int32_t *roots, int32_t k,
                   int32_t *depths_host, ent_batch_stats *stats);
 
@@ -72,9 +80,11 @@ int ent_bfs_batch(ent_graph *g, const int32_t *roots, int32_t k,
 int ent_device_alloc(ent_graph *g, int64_t bytes, void **out);
 int ent_device_free(ent_graph *g, void *p);
 
-/* 64-bit FNV-1a hash of each of `rows` rows of `cols` int32 values in device
- * memory, computed on the device (for comparing large depth arrays without
- * copying them). */
+/* Position-keyed 64-bit checksum of each of `rows` rows of `cols` int32
+ * values in device memory, computed on the device (for comparing large depth
+ * arrays without copying them): sum over i, mod 2^64, of the splitmix64
+ * finaliser of ((i << 32) ^ value_i) + 0x9E3779B97F4A7C15. Uses a buffer of
+ * the handle (no allocation per call). */
 int ent_device_row_hash(ent_graph *g, const int32_t *dev, int64_t rows,
                         int64_t cols, uint64_t *hashes_host);
 

@@ -86,20 +86,77 @@ def test_batch_closed_forms(oracle, name):
             assert np.array_equal(out[i], np.asarray(depth_fn(int(r)), dtype=np.int32)), (name, r)
 
 
-def test_batch_level_capacity_error(rmat16):
-    g, dg = rmat16
-    path, _ = graphs.path(300)
-    with DeviceGraph(path) as dp:
+@pytest.mark.parametrize("launch", ["stream", "persistent"])
+def test_batch_beyond_level_capacity(oracle, launch):
+    """A search deeper than the reserved levels still returns every row: the
+    sources that finished within the reserved levels come from the batch,
+    the unfinished ones are searched again on the single-source path."""
+    form = {"stream": L.LAUNCH_STREAM, "persistent": L.LAUNCH_PERSISTENT}[launch]
+    p300, pdepth = graphs.path(300)
+    with DeviceGraph(p300) as dp:
+        dp.configure(launch=form)
         dp.batch_reserve(4, max_levels=10)
-        with pytest.raises(L.EntError) as e:
-            dp.bfs_batch([0, 1, 2, 3])
-        assert e.value.code == L.ENT_ENOMEM and "levels" in str(e.value)
-        dp.batch_reserve(4, max_levels=64)
-        with pytest.raises(L.EntError):
-            dp.bfs_batch([0, 299])          # needs 300 levels
+        _say(f"[gpu] batch beyond capacity: path300 {launch}")
+        out, st = dp.bfs_batch([0, 1, 2, 299])
+        for i, r in enumerate([0, 1, 2, 299]):
+            assert np.array_equal(out[i], np.asarray(pdepth(r), dtype=np.int32)), r
+        assert st["fallback_rows"] == 4 and st["levels"] == 300
         dp.batch_reserve(4, max_levels=250)
-        out, _ = dp.bfs_batch([100, 150])      # 200 levels
-        assert out[0][299] == 199 and out[1][0] == 150
+        out, st = dp.bfs_batch([100, 150])      # 200 levels fit
+        assert out[0][299] == 199 and out[1][0] == 150 and st["fallback_rows"] == 0
+    # One deep source among 63 shallow ones: a 400-vertex path next to a
+    # 2,000-leaf star; only the path end's row is searched again.
+    n1, n2 = 400, 2000
+    e = [(i, i + 1) for i in range(n1 - 1)] + [(n1, n1 + j) for j in range(1, n2)]
+    g = graphs.edges_graph(n1 + n2, e)
+    roots = np.r_[np.arange(n1 + 1, n1 + 64), 0].astype(np.int32)
+    with DeviceGraph(g) as dg:
+        dg.configure(launch=form)
+        dg.batch_reserve(64, max_levels=40)
+        _say(f"[gpu] batch beyond capacity: path+star {launch}")
+        out, st = dg.bfs_batch(roots)
+        assert st["fallback_rows"] == 1, st
+        for i, r in enumerate(roots):
+            assert np.array_equal(out[i], oracle.bfs(g.beg_pos, g.adj, int(r))), (i, r)
+        deg = g.degrees()
+        reached = out >= 0
+        assert st["reached_vertices"] == int(reached.sum())
+        assert st["reached_edges"] == int((reached * deg[None, :]).sum())
+        assert st["levels"] == n1
+
+
+def test_batch_unaligned_output(rmat16):
+    """Rows written to an output pointer 4 bytes past a 16-byte boundary (the
+    scalar-store path) equal those written to an aligned one."""
+    g, dg = rmat16
+    roots = g.pick_roots(32, 77)
+    ptr = dg.device_alloc(32 * g.n * 4 + 64)
+    try:
+        _say("[gpu] batch output alignment")
+        dg.bfs_batch_device(roots, ptr + 16)
+        want = dg.row_hash(ptr + 16, 32, g.n)
+        for off in (4, 8, 12):
+            dg.bfs_batch_device(roots, ptr + off)
+            assert np.array_equal(dg.row_hash(ptr + off, 32, g.n), want), off
+        d, _ = dg.bfs(int(roots[3]))
+        assert want[3] == row_hash_host(d)
+    finally:
+        dg.device_free(ptr)
+
+
+def test_row_hash_many_rows(rmat16):
+    """Checksums of more rows than one launch sums (512 per launch)."""
+    g, dg = rmat16
+    roots = g.pick_roots(32, 78)
+    ptr = dg.device_alloc(32 * g.n * 4)
+    try:
+        dg.bfs_batch_device(roots, ptr)
+        host = dg.copy_depth(ptr, 32 * g.n)
+        rows, cols = 1100, (32 * g.n) // 1100
+        got = dg.row_hash(ptr, rows, cols)
+        assert np.array_equal(got, row_hash_host(host[:rows * cols].reshape(rows, cols)))
+    finally:
+        dg.device_free(ptr)
 
 
 def test_batch_device_hash(rmat16):

@@ -1,16 +1,27 @@
-"""Parity at the benchmark sizes: R-MAT scale 22 (64-root workload) against
-the oracle bit for bit, the 512-root multi-source search on R-MAT scale 23
-against the single-source path on every row (device row checksums) and the
-oracle on sampled rows, and the 4096 x 4096 grid against the oracle."""
+"""Parity at the benchmark sizes, under the configurations the benchmark runs:
+R-MAT scale 22 (64-root workload, tuned single-source configuration) against
+the oracle bit for bit; the 512-root multi-source search on R-MAT scale 23
+(tuned multi-source knobs) against the single-source path on every row and
+the oracle on sampled rows; the 4096 x 4096 grid in every launch form against
+the oracle; a complete bipartite graph with more than 2^31 directed edges
+(int64 offsets end to end) in every launch form and a 64-source batch; and
+the serving workload (R-MAT 26 + R-MAT 20, Poisson arrivals) with every query
+checked against the single-source path and sampled queries against the
+oracle."""
 import sys
+import time
 
 import numpy as np
 import pytest
 
-from enterprise_amd import DeviceGraph, HostGraph
+from enterprise_amd import DeviceGraph, HostGraph, autotune
+from enterprise_amd import _lib as L
 from enterprise_amd.bfs import row_hash_host
 
 pytestmark = pytest.mark.gpu
+
+# Knobs of a saved configuration that the benchmark does not take from it.
+_SKIP = ("reserved", "launch", "hub_cache")
 
 
 def _say(m):
@@ -18,13 +29,21 @@ def _say(m):
     sys.stdout.flush()
 
 
+def _tuned(dg, name, keys=None):
+    cfg = autotune.load(name)
+    assert cfg is not None, name
+    dg.configure(**{k: v for k, v in cfg.items() if k not in _SKIP and (keys is None or k in keys)})
+    return cfg
+
+
 def test_rmat22_roots_bit_exact(oracle):
     g = HostGraph.rmat(22, 16, seed=22)
     roots = g.pick_roots(64, 1002)
     with DeviceGraph(g) as dg:
-        for i in (0, 17, 63):
+        _tuned(dg, "rmat22")
+        for i in (0, 9, 17, 26, 35, 44, 53, 63):
             r = int(roots[i])
-            _say(f"[gpu] rmat22 root {r}")
+            _say(f"[gpu] rmat22 (tuned) root {i}: {r}")
             got, st = dg.bfs(r)
             want = oracle.bfs(g.beg_pos, g.adj, r)
             assert np.array_equal(got, want), r
@@ -45,19 +64,23 @@ def test_rmat23_batch_512_rows(oracle):
     g = HostGraph.rmat(23, 16, seed=23)
     roots = g.pick_roots_mixed(512, 256, 1004)
     with DeviceGraph(g) as dg:
+        _tuned(dg, "rmat22")                               # the plain path, as the benchmark runs it
         k = len(roots)
+        plain = np.empty(k, dtype=np.uint64)
+        _say("[gpu] rmat23 plain 512 roots")
+        for i, r in enumerate(roots):
+            dg.bfs_device(int(r))
+            plain[i] = dg.row_hash(dg.depth_device_ptr(), 1, g.n)[0]
+        _tuned(dg, "rmat23_batch", autotune.BATCH_SPACE)   # multi-source knobs
         out = dg.device_alloc(k * g.n * 4)
         try:
             dg.batch_reserve(k, 48)
             _say("[gpu] rmat23 batch 512")
             st = dg.bfs_batch_device(roots, out)
             h = dg.row_hash(out, k, g.n)
-            plain = np.empty(k, dtype=np.uint64)
-            for i, r in enumerate(roots):
-                dg.bfs_device(int(r))
-                plain[i] = dg.row_hash(dg.depth_device_ptr(), 1, g.n)[0]
             assert np.array_equal(h, plain)
             assert st["bu_level_mask"] & st["td_level_mask"]   # some level split its words
+            assert st["fallback_rows"] == 0
             for i in (0, 255, 256, 511):
                 want = oracle.bfs(g.beg_pos, g.adj, int(roots[i]))
                 assert int(h[i]) == int(row_hash_host(want)), i
@@ -68,15 +91,136 @@ def test_rmat23_batch_512_rows(oracle):
             dg.device_free(out)
 
 
-def test_grid4096_root(oracle):
+def test_grid4096_forms(oracle):
+    """One root per launch form against the oracle (more than 4,000 levels
+    each), and every root through the device all-vertex check."""
     g = HostGraph.grid(4096, 4096, delete_ppm=20000, seed=4096, keep_lcc=True)
-    r = int(g.pick_roots(16, 1003)[3])
+    roots = g.pick_roots(16, 1003)
+    forms = {"stream": L.LAUNCH_STREAM, "graph": L.LAUNCH_GRAPH, "persistent": L.LAUNCH_PERSISTENT,
+             "levels": L.LAUNCH_LEVELS}
     with DeviceGraph(g) as dg:
-        _say(f"[gpu] grid4096 root {r}")
-        got, st = dg.bfs(r)
-        want = oracle.bfs(g.beg_pos, g.adj, r)
-        assert np.array_equal(got, want)
-        assert st["levels"] == int(want.max()) + 1 and st["levels"] > 4000
-        for r2 in g.pick_roots(16, 1003):
+        for j, (name, form) in enumerate(forms.items()):
+            r = int(roots[3 + j])
+            dg.configure(launch=form, graph_levels=96)
+            _say(f"[gpu] grid4096 {name} root {r}")
+            got, st = dg.bfs(r)
+            want = oracle.bfs(g.beg_pos, g.adj, r)
+            assert np.array_equal(got, want), name
+            assert st["levels"] == int(want.max()) + 1 and st["levels"] > 4000
+        dg.configure(launch=L.LAUNCH_STREAM)
+        for r2 in roots:
             dg.bfs_device(int(r2))
             assert dg.validate_device(int(r2)) == [0, 0, 0, 0], int(r2)
+
+
+def _bipartite(a):
+    """K_{a,a}: side A = 0..a-1, side B = a..2a-1, every row sorted."""
+    n = 2 * a
+    beg = np.arange(n + 1, dtype=np.int64) * a
+    adj = np.empty(n * a, dtype=np.int32)
+    half = a * a
+    adj[:half].reshape(a, a)[:] = np.arange(a, n, dtype=np.int32)[None, :]
+    adj[half:].reshape(a, a)[:] = np.arange(0, a, dtype=np.int32)[None, :]
+    return beg, adj
+
+
+def test_offsets_past_int32():
+    """Complete bipartite K_{33000,33000}: m = 2,178,000,000 directed edges,
+    past int32. From a root on side A, depth 0 at the root, 2 on the rest of
+    A and 1 on B; every reached vertex counts all its edges. All four launch
+    forms, forced directions, the device check, parents and a 64-source
+    batch."""
+    a = 33000
+    n, m = 2 * a, 2 * a * a
+    assert m > 2 ** 31
+    t0 = time.perf_counter()
+    beg, adj = _bipartite(a)
+    _say(f"[gpu] K_{{{a},{a}}}: n={n} m={m} built in {time.perf_counter() - t0:.1f}s")
+
+    def want(r):
+        d = np.full(n, 2 if r < a else 1, dtype=np.int32)
+        d[a:] = 1 if r < a else 2
+        d[r] = 0
+        return d
+
+    with DeviceGraph(beg_pos=beg, adj=adj) as dg:
+        del adj
+        _say("[gpu] K_{a,a} uploaded")
+        forms = {"stream": L.LAUNCH_STREAM, "graph": L.LAUNCH_GRAPH, "persistent": L.LAUNCH_PERSISTENT,
+                 "levels": L.LAUNCH_LEVELS}
+        dirs = {"auto": L.DIR_AUTO, "topdown": L.DIR_TOPDOWN, "bottomup": L.DIR_BOTTOMUP,
+                "reference": L.DIR_REFERENCE}
+        cases = [(f, "auto") for f in forms] + [("stream", d) for d in ("topdown", "bottomup", "reference")]
+        for j, (f, d) in enumerate(cases):
+            r = (12345 + 7919 * j) % n
+            dg.configure(launch=forms[f], direction=dirs[d])
+            _say(f"[gpu] K_{{a,a}} {f}/{d} root {r}")
+            got, st = dg.bfs(r)
+            assert np.array_equal(got, want(r)), (f, d, r)
+            assert st["reached_edges"] == m and st["reached_vertices"] == n, st
+            assert st["levels"] == 3
+            assert dg.validate_device(r) == [0, 0, 0, 0]
+        dg.configure(launch=L.LAUNCH_STREAM, direction=L.DIR_AUTO)
+        r = 40000
+        d, p, _ = dg.bfs_parents(r)
+        assert np.array_equal(d, want(r))
+        assert p[r] == r and np.all(p[:a] == r)                  # side A hangs off the root
+        others = np.r_[np.arange(a, r), np.arange(r + 1, n)]
+        assert np.all((p[others] >= 0) & (p[others] < a))        # the rest of B via side A
+        roots = np.r_[np.arange(0, n, n // 64)[:64]].astype(np.int32)
+        _say("[gpu] K_{a,a} 64-source batch")
+        out, st = dg.bfs_batch(roots)
+        for i, r in enumerate(roots):
+            assert np.array_equal(out[i], want(int(r))), (i, int(r))
+        assert st["reached_edges"] == 64 * m and st["fallback_rows"] == 0
+
+
+def test_serving_rmat26_rmat20(oracle):
+    """The serving workload: R-MAT 26 (seed 26) and R-MAT 20 (seed 20)
+    resident, a seeded Poisson stream at 500 queries/s for one second, 80% to
+    the large graph, uniform non-isolated roots. Every query's row checksum
+    equals the single-source result for its root, and sampled queries (2 on
+    R-MAT 26, 4 on R-MAT 20) equal the CPU oracle bit for bit."""
+    from enterprise_amd.serving import BFSServer, run_load
+
+    t0 = time.perf_counter()
+    gb = HostGraph.rmat(26, 16, seed=26)
+    _say(f"[gpu] serving: R-MAT 26 n={gb.n} m={gb.m} ({time.perf_counter() - t0:.1f}s)")
+    gs = HostGraph.rmat(20, 16, seed=20)
+    assert gb.m > 2_000_000_000
+    roots = {"large": gb.pick_roots(4096, 1005), "small": gs.pick_roots(4096, 1006)}
+    with DeviceGraph(gb) as db, DeviceGraph(gs) as ds:
+        for d in (db, ds):
+            _tuned(d, "rmat22")
+        _say("[gpu] serving: graphs resident")
+        srv = BFSServer(max_batch=64, max_levels=28, max_slots=3, plain_max=12)
+        srv.add_graph("large", db, slots=3)
+        srv.add_graph("small", ds, slots=2, copy_rows=True)
+        srv.start()
+        try:
+            res = run_load(srv, 500, 1.0, 1505, roots, {"large": 0.8, "small": 0.2})
+        finally:
+            srv.stop()
+        _say(f"[gpu] serving: {len(res)} queries served; checking each on the single-source path")
+        try:
+            by = {"large": [q for q in res if q.graph == "large"], "small": [q for q in res if q.graph == "small"]}
+            assert len(by["large"]) >= 2 and len(by["small"]) >= 4
+            sizes = srv.lanes["large"].stats["batch_sizes"] + srv.lanes["small"].stats["batch_sizes"]
+            assert sum(sizes) == len(res)
+            for q in res:
+                d = db if q.graph == "large" else ds
+                d.bfs_device(q.root)
+                assert int(d.row_hash(d.depth_device_ptr(), 1, d.n)[0]) == q.row_hash, (q.graph, q.root)
+            # Sampled against the oracle: the large graph's rows by checksum
+            # (a host copy of each is 268 MB), the small graph's row by row.
+            multi = [q for q in by["large"] if q.batch_size > srv.plain_max]
+            single = [q for q in by["large"] if q.batch_size <= srv.plain_max]
+            pick = multi[:1] + single[:1] if multi and single else by["large"][:2]
+            for q in pick:
+                _say(f"[gpu] serving: oracle R-MAT 26 root {q.root} (batch of {q.batch_size})")
+                want = oracle.bfs(gb.beg_pos, gb.adj, q.root)
+                assert q.row_hash == int(row_hash_host(want)), q.root
+            for q in by["small"][:4]:
+                assert np.array_equal(q.depth, oracle.bfs(gs.beg_pos, gs.adj, q.root)), q.root
+        finally:
+            srv.close()

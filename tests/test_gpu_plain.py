@@ -272,9 +272,11 @@ def test_level_trace(oracle, rmat16):
     base = dg.config
     deg = np.diff(g.beg_pos)
     try:
-        for launch in (L.LAUNCH_STREAM, L.LAUNCH_PERSISTENT):
+        # The hipGraph form replays 6 launches at a time, so a search spans
+        # several replays whose launch arguments restart at 0.
+        for launch in (L.LAUNCH_STREAM, L.LAUNCH_PERSISTENT, L.LAUNCH_GRAPH):
             for d in (L.DIR_AUTO, L.DIR_TOPDOWN):
-                dg.configure(count_examined=1, launch=launch, direction=d)
+                dg.configure(count_examined=1, launch=launch, direction=d, graph_levels=6)
                 r = int(g.pick_roots(1, 55)[0])
                 st = _check(oracle, g, dg, r, f"trace/{launch}/{d}")
                 recs = dg.level_trace()

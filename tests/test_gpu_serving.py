@@ -30,7 +30,7 @@ def two_graphs():
     db.free()
 
 
-def test_alone_equals_mix(two_graphs):
+def test_alone_equals_mix(oracle, two_graphs):
     srv = BFSServer(max_batch=64, max_levels=40, max_slots=3)
     for name, (hg, dg) in two_graphs.items():
         srv.add_graph(name, dg, copy_rows=True)
@@ -53,6 +53,14 @@ def test_alone_equals_mix(two_graphs):
             assert q.row_hash == int(row_hash_host(alone[key]))
         sizes = srv.lanes["big"].stats["batch_sizes"]
         assert max(sizes) > srv.plain_max  # the multi-source path served some batches
+        # Sampled queries against the CPU oracle, among them queries served by
+        # multi-source batches.
+        multi = [q for q in burst + mixed if q.batch_size > srv.plain_max]
+        sample = multi[:3] + [q for q in burst + mixed if q.graph == "small"][:3]
+        assert len(sample) == 6
+        for q in sample:
+            hg, _ = two_graphs[q.graph]
+            assert np.array_equal(q.depth, oracle.bfs(hg.beg_pos, hg.adj, q.root)), (q.graph, q.root)
     finally:
         srv.close()
 
@@ -114,38 +122,62 @@ def test_memory_budget_admission(two_graphs):
     srv.close()
 
 
-def test_failed_batch_keeps_slot_usable():
-    """A batch that exceeds the slot's level capacity fails its queries with
-    an error, and the same slot then serves the next batch correctly."""
-    g, depth_fn = graphs.path(60)
+def test_deep_query_in_batch(oracle):
+    """One query deeper than the slot's reserved levels (the end of a
+    400-vertex path) served in one 64-query multi-source batch with 63
+    shallow ones (leaves of a star): every query gets its correct depths."""
+    n1, n2 = 400, 3000
+    e = [(i, i + 1) for i in range(n1 - 1)] + [(n1, n1 + j) for j in range(1, n2)]
+    g = graphs.edges_graph(n1 + n2, e)
     with DeviceGraph(g) as dg:
-        srv = BFSServer(max_batch=4, max_levels=40, max_slots=1, plain_max=0)
-        srv.add_graph("p", dg, copy_rows=True)
-        srv.start()
+        srv = BFSServer(max_batch=64, max_levels=24, max_slots=1, plain_max=0)
+        srv.add_graph("g", dg, copy_rows=True)
         try:
-            bad = srv.submit("p", 0)                  # 60 levels > 40
-            with pytest.raises(L.EntError):
-                bad.wait(60)
-            ok = srv.submit("p", 30).wait(60)         # 31 levels fit
-            assert np.array_equal(ok.depth, np.asarray(depth_fn(30), dtype=np.int32))
+            roots = [n1 + 1 + i for i in range(63)]
+            roots.insert(17, 0)
+            qs = [srv.submit("g", r) for r in roots]   # queued before start: one batch
+            _say("[gpu] serving: deep query in a 64-query batch")
+            srv.start()
+            for q in qs:
+                q.wait(120)
+            st = srv.lanes["g"].stats
+            assert st["batch_sizes"] == [64] and st["fallback_rows"] == 1, st
+            for q in qs:
+                want = oracle.bfs(g.beg_pos, g.adj, q.root)
+                assert np.array_equal(q.depth, want), q.root
+                assert q.row_hash == int(row_hash_host(want))
         finally:
             srv.close()
+
+
+def test_failed_batch_keeps_slot_usable(monkeypatch):
+    """A batch whose search fails passes the error to each of its queries,
+    and the same slot then serves the next batch correctly; a root outside
+    the graph is refused at submit and joins no batch."""
     h = HostGraph.rmat(10, 8, seed=5)
     with DeviceGraph(h) as dh:
         r = int(h.pick_roots(1, 2)[0])
-        srv = BFSServer(max_batch=4, max_levels=3, max_slots=1, plain_max=0)
+        srv = BFSServer(max_batch=4, max_levels=32, max_slots=1, plain_max=0)
         srv.add_graph("r", dh, copy_rows=True)
+        slot_dg = srv.lanes["r"].slots[0].dg
+        real = slot_dg.bfs_batch_device
+        calls = {"n": 0}
+
+        def failing_once(roots, out):
+            calls["n"] += 1
+            if calls["n"] == 1:
+                raise L.EntError(L.ENT_EDEVICE, "injected failure")
+            return real(roots, out)
+
+        monkeypatch.setattr(slot_dg, "bfs_batch_device", failing_once)
+        with pytest.raises(ValueError):
+            srv.submit("r", h.n)
         srv.start()
         try:
             with pytest.raises(L.EntError):
-                srv.submit("r", r).wait(60)           # R-MAT scale 10 needs > 3 levels
-        finally:
-            srv.close()
-        srv = BFSServer(max_batch=4, max_levels=32, max_slots=1, plain_max=0)
-        srv.add_graph("r", dh, copy_rows=True)
-        srv.start()
-        try:
+                srv.submit("r", r).wait(60)
             q = srv.submit("r", r).wait(60)
             assert np.array_equal(q.depth, dh.bfs(r)[0])
+            assert calls["n"] == 2
         finally:
             srv.close()
