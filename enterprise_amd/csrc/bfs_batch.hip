@@ -648,7 +648,14 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
 #pragma unroll
       for (int w = 0; w < W; ++w) x.w[w] |= xp.w[w];
     }
-    if (valid) store_mask<W>(nx, v, x);
+    // lv[L + 1] already holds zero, or the bits this level pushed to v:
+    // store only when the pull found new bits.
+    {
+      bool pulled = false;
+#pragma unroll
+      for (int w = 0; w < W; ++w) pulled |= (acc.w[w] & tgt.w[w]) != 0;
+      if (valid && pulled) store_mask<W>(nx, v, x);
+    }
     bool any;
     settle_bu<W>(a, L, valid, v, ftc, x, e - s, jw, mf, alg, munf, act, mfw, any);
     const unsigned long long bits = __ballot(any);

@@ -57,6 +57,15 @@ int32_t ent_oracle_bfs_parents(int64_t n, const int64_t *beg_pos,
   return maxd + 1;
 }
 
+/* Direction-optimizing BFS (Beamer, Asanovic, Patterson, SC'12) on OpenMP
+ * threads: the CPU baseline beside the GPU path, and a second CPU BFS the
+ * tests compare with ent_oracle_bfs. Top-down levels expand a queue, each
+ * thread claiming neighbours with a compare-and-swap on their depth and
+ * collecting them in its own buffer (one shared reservation per thread and
+ * level); bottom-up levels sweep the unvisited vertices against a frontier
+ * bitmap (one bit per vertex), each thread writing whole 64-vertex words of
+ * the next bitmap. The queue of a bottom-up level's output is rebuilt, in
+ * parallel, only when the next level runs top-down. alpha = 15, beta = 18. */
 int32_t ent_oracle_bfs_do(int64_t n, const int64_t *beg_pos, const int32_t *adj,
                           int32_t root, int32_t *depth, int32_t threads) {
   if (n <= 0 || root < 0 || root >= n || !beg_pos || !depth) return -1;
@@ -66,11 +75,18 @@ int32_t ent_oracle_bfs_do(int64_t n, const int64_t *beg_pos, const int32_t *adj,
   threads = 1;
 #endif
   const int64_t m = beg_pos[n];
+  const int64_t nw = (n + 63) / 64;
   int32_t *cur = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
   int32_t *nxt = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
-  if (!cur || !nxt) {
+  uint64_t *fb = (uint64_t *)calloc((size_t)nw, sizeof(uint64_t)); /* frontier bits */
+  uint64_t *nb = (uint64_t *)calloc((size_t)nw, sizeof(uint64_t)); /* next frontier bits */
+  int64_t *tcount = (int64_t *)calloc((size_t)threads + 1, sizeof(int64_t));
+  if (!cur || !nxt || !fb || !nb || !tcount) {
     free(cur);
     free(nxt);
+    free(fb);
+    free(nb);
+    free(tcount);
     return -1;
   }
   int64_t i;
@@ -79,55 +95,107 @@ int32_t ent_oracle_bfs_do(int64_t n, const int64_t *beg_pos, const int32_t *adj,
   depth[root] = 0;
   cur[0] = root;
   int64_t ncur = 1;
+  int queue_ok = 1; /* cur holds the frontier of `level` */
+  int bits_ok = 0;  /* fb holds it */
   int64_t mf = beg_pos[root + 1] - beg_pos[root];
   int64_t mu = m - mf;
   int bottom_up = 0;
   int32_t level = 0;
   const int64_t alpha = 15, beta = 18;
+  const int64_t chunk = 64 * 64; /* bottom-up work unit: 64 whole bitmap words */
   while (ncur > 0) {
     if (!bottom_up && mf * alpha > mu) bottom_up = 1;
     else if (bottom_up && ncur * beta < n) bottom_up = 0;
     int64_t nnext = 0, mnext = 0;
     if (bottom_up) {
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 4096) reduction(+ : nnext, mnext)
-      for (i = 0; i < n; ++i) {
-        if (depth[i] >= 0) continue;
-        for (int64_t k = beg_pos[i]; k < beg_pos[i + 1]; ++k)
-          if (depth[adj[k]] == level) {
-            depth[i] = level + 1;
-            nnext += 1;
-            mnext += beg_pos[i + 1] - beg_pos[i];
-            break;
-          }
+      if (!bits_ok) { /* frontier queue -> bitmap */
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (i = 0; i < nw; ++i) fb[i] = 0;
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (i = 0; i < ncur; ++i) __sync_fetch_and_or(&fb[cur[i] >> 6], 1ull << (cur[i] & 63));
       }
-      /* Rebuild the queue of the new frontier (needed if we switch back). */
-      int64_t w = 0;
-      for (i = 0; i < n; ++i)
-        if (depth[i] == level + 1) nxt[w++] = (int32_t)i;
-      nnext = w;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) reduction(+ : nnext, mnext)
+      for (i = 0; i < (n + chunk - 1) / chunk; ++i) {
+        const int64_t v0 = i * chunk, v1 = v0 + chunk < n ? v0 + chunk : n;
+        for (int64_t w = v0 >> 6; w <= (v1 - 1) >> 6; ++w) {
+          uint64_t word = 0;
+          const int64_t e = (w + 1) * 64 < v1 ? (w + 1) * 64 : v1;
+          for (int64_t v = w * 64; v < e; ++v) {
+            if (depth[v] >= 0) continue;
+            for (int64_t k = beg_pos[v]; k < beg_pos[v + 1]; ++k) {
+              const int32_t u = adj[k];
+              if ((fb[u >> 6] >> (u & 63)) & 1ull) {
+                depth[v] = level + 1;
+                word |= 1ull << (v & 63);
+                nnext += 1;
+                mnext += beg_pos[v + 1] - beg_pos[v];
+                break;
+              }
+            }
+          }
+          nb[w] = word;
+        }
+      }
+      uint64_t *t = fb;
+      fb = nb;
+      nb = t;
+      bits_ok = 1;
+      queue_ok = 0;
     } else {
+      if (!queue_ok) { /* bitmap -> queue: per-thread counts, prefix, fill */
+#pragma omp parallel num_threads(threads)
+        {
+          int tid = 0, nt = 1;
+#ifdef _OPENMP
+          tid = omp_get_thread_num();
+          nt = omp_get_num_threads();
+#endif
+          const int64_t w0 = nw * tid / nt, w1 = nw * (tid + 1) / nt;
+          int64_t c = 0;
+          for (int64_t w = w0; w < w1; ++w) c += __builtin_popcountll(fb[w]);
+          tcount[tid + 1] = c;
+#pragma omp barrier
+#pragma omp single
+          for (int k = 0; k < nt; ++k) tcount[k + 1] += tcount[k];
+          int64_t o = tcount[tid];
+          for (int64_t w = w0; w < w1; ++w)
+            for (uint64_t x = fb[w]; x; x &= x - 1) cur[o++] = (int32_t)(w * 64 + __builtin_ctzll(x));
+        }
+        queue_ok = 1;
+      }
       int64_t w = 0;
 #pragma omp parallel num_threads(threads) reduction(+ : mnext)
       {
-#pragma omp for schedule(dynamic, 64)
+        int32_t buf[1024];
+        int nbuf = 0;
+#pragma omp for schedule(dynamic, 64) nowait
         for (i = 0; i < ncur; ++i) {
           int32_t u = cur[i];
           for (int64_t k = beg_pos[u]; k < beg_pos[u + 1]; ++k) {
             int32_t v = adj[k];
-            if (depth[v] < 0 &&
-                __sync_bool_compare_and_swap(&depth[v], -1, level + 1)) {
-              int64_t slot = __sync_fetch_and_add(&w, 1);
-              nxt[slot] = v;
+            if (depth[v] < 0 && __sync_bool_compare_and_swap(&depth[v], -1, level + 1)) {
               mnext += beg_pos[v + 1] - beg_pos[v];
+              buf[nbuf++] = v;
+              if (nbuf == 1024) {
+                const int64_t slot = __sync_fetch_and_add(&w, (int64_t)nbuf);
+                memcpy(nxt + slot, buf, sizeof(int32_t) * (size_t)nbuf);
+                nbuf = 0;
+              }
             }
           }
         }
+        if (nbuf) {
+          const int64_t slot = __sync_fetch_and_add(&w, (int64_t)nbuf);
+          memcpy(nxt + slot, buf, sizeof(int32_t) * (size_t)nbuf);
+        }
       }
       nnext = w;
+      int32_t *t = cur;
+      cur = nxt;
+      nxt = t;
+      queue_ok = 1;
+      bits_ok = 0;
     }
-    int32_t *t = cur;
-    cur = nxt;
-    nxt = t;
     ncur = nnext;
     mu -= mnext;
     mf = mnext;
@@ -135,6 +203,9 @@ int32_t ent_oracle_bfs_do(int64_t n, const int64_t *beg_pos, const int32_t *adj,
   }
   free(cur);
   free(nxt);
+  free(fb);
+  free(nb);
+  free(tcount);
   return level;
 }
 
