@@ -144,6 +144,8 @@ SIGNATURES = [
     ("ent_device_free", C.c_int, [C.c_void_p, C.c_void_p]),
     ("ent_device_row_hash", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
                                       C.POINTER(C.c_uint64)]),
+    ("ent_device_row_hash_noalloc", C.c_int, [C.c_int32, C.c_void_p, C.c_int64, C.c_int64,
+                                              C.POINTER(C.c_uint64)]),
     ("ent_bfs_validate_device", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, i64p]),
     ("ent_last_device_ms", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("ent_tuples_read", C.c_int, [C.c_char_p, C.c_int64, C.POINTER(Tuples)]),

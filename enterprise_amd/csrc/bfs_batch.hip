@@ -894,6 +894,20 @@ __global__ void k_row_hash(const int *dev, long long cols, long long chunk, unsi
   if (threadIdx.x == 0) atomicAdd(&out[r], red[0] + red[1] + red[2] + red[3]);
 }
 
+// Scratch of ent_device_row_hash_noalloc: one checksum accumulator per row,
+// ENT_NOALLOC_HASH_ROWS rows. A file-scope device array: each GPU the library
+// runs on holds its own copy until the process ends, at its own address
+// (looked up once per device with hipGetSymbolAddress; host code never uses
+// the name itself as a device address).
+static __device__ __align__(256) unsigned long long scratch[ENT_NOALLOC_HASH_ROWS];
+
+// Clears the first n words of p (a kernel, not hipMemsetAsync, so the
+// entry point adds no memset node when a caller's stream is captured).
+__global__ void k_zero_u64(unsigned long long *p, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
 struct BatchWorkspace {
   int words = 0;      // W (1, 2, 4 or 8)
   int lcap = 0;       // max joint levels
@@ -1290,3 +1304,62 @@ int ent_device_row_hash(ent_graph *g, const int32_t *dev, int64_t rows, int64_t 
 }
 
 }  // extern "C"
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];         // one call at a time per device
+unsigned long long *g_scratch[kMaxDevices];  // device address of `scratch` per device
+
+// Device address of this device's `scratch`, looked up on the first call on
+// the device (the caller has made it current and holds its lock).
+int scratch_of(int device, unsigned long long **out) {
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      ignore(hipGetLastError());
+      return set_error(ENT_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    }
+    g_scratch[device] = static_cast<unsigned long long *>(p);
+  }
+  *out = g_scratch[device];
+  return ENT_OK;
+}
+
+}  // namespace
+
+extern "C" int ent_device_row_hash_noalloc(int32_t device, const int32_t *dev, int64_t rows, int64_t cols,
+                                           uint64_t *hashes_host) {
+  clear_error();
+  if (!dev || !hashes_host || rows < 1 || cols < 1) return set_error(ENT_EINVAL, "bad argument");
+  if (rows > ENT_NOALLOC_HASH_ROWS)
+    return set_error(ENT_EINVAL, "rows=%lld exceeds the %d rows of the no-allocation scratch", (long long)rows,
+                     ENT_NOALLOC_HASH_ROWS);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess) {
+    ignore(hipGetLastError());
+    return set_error(ENT_EDEVICE, "no HIP device available");
+  }
+  if (device < 0 || device >= ndev || device >= kMaxDevices)
+    return set_error(ENT_EINVAL, "device %d of %d", device, ndev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu[device]);
+  HIP_TRY(hipSetDevice(device));
+  unsigned long long *acc = nullptr;
+  const int rc = scratch_of(device, &acc);
+  if (rc) return rc;
+  // The accumulators keep the previous call's sums: clear the ones this call
+  // uses, then sum and copy on the null stream (the copy waits for both).
+  const long long nr = rows;
+  hipLaunchKernelGGL(k_zero_u64, dim3(unsigned((nr + 255) / 256)), dim3(256), 0, 0, acc, nr);
+  HIP_TRY(hipGetLastError());
+  long long chunk = 1 << 16;
+  const long long nchunks = (cols + chunk - 1) / chunk;
+  long long ncols = cols;
+  void *args[] = {const_cast<int32_t **>(&dev), &ncols, &chunk, &acc};
+  HIP_TRY(hipLaunchKernel(reinterpret_cast<const void *>(&k_row_hash), dim3(unsigned(nchunks), unsigned(nr)),
+                          dim3(256), args, 0, nullptr));
+  HIP_TRY(hipMemcpy(hashes_host, acc, sizeof(unsigned long long) * size_t(nr), hipMemcpyDeviceToHost));
+  return ENT_OK;
+}
+

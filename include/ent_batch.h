@@ -88,6 +88,25 @@ int ent_device_free(ent_graph *g, void *p);
 int ent_device_row_hash(ent_graph *g, const int32_t *dev, int64_t rows,
                         int64_t cols, uint64_t *hashes_host);
 
+/* Rows one ent_device_row_hash_noalloc call can sum. */
+#define ENT_NOALLOC_HASH_ROWS 4096
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the checksums of ent_device_row_hash, for at most
+ * ENT_NOALLOC_HASH_ROWS rows, with no graph handle and no allocation of any
+ * kind. The per-row sums live in a fixed device array of
+ * ENT_NOALLOC_HASH_ROWS * 8 bytes (32 KiB) that each GPU holds from the
+ * first call on it until the process ends; every call on a GPU shares it,
+ * so calls on one device from one process are serialized (each clears the
+ * sums it uses). Runs on `device`'s null stream and returns once
+ * hashes_host is written; the rows in `dev` must be complete (work queued
+ * on other streams is not waited for). More rows than the maximum:
+ * ENT_EINVAL. */
+int ent_device_row_hash_noalloc(int32_t device, const int32_t *dev,
+                                int64_t rows, int64_t cols,
+                                uint64_t *hashes_host);
+
 #ifdef __cplusplus
 }
 #endif
