@@ -123,6 +123,25 @@ template <int W>
 __device__ __forceinline__ Mask<W> load_mask(const unsigned long long *base, long long v) {
   return *reinterpret_cast<const Mask<W> *>(base + v * W);
 }
+// Gather of v's mask for the pull: with H >= 0 (W = 8) only words 4H..4H+3
+// (one 32-byte half), the rest zero; with H < 0 the whole mask.
+template <int W, int H>
+__device__ __forceinline__ Mask<W> gather_mask(const unsigned long long *base, long long v) {
+  if constexpr (W == 8 && H >= 0) {
+    typedef unsigned long long u64x4 __attribute__((ext_vector_type(4)));
+    const u64x4 q = *reinterpret_cast<const u64x4 *>(base + v * W + 4 * H);
+    Mask<W> m;
+#pragma unroll
+    for (int w = 0; w < W; ++w) m.w[w] = 0;
+    m.w[4 * H + 0] = q.x;
+    m.w[4 * H + 1] = q.y;
+    m.w[4 * H + 2] = q.z;
+    m.w[4 * H + 3] = q.w;
+    return m;
+  } else {
+    return load_mask<W>(base, v);
+  }
+}
 template <int W>
 __device__ __forceinline__ void store_mask(unsigned long long *base, long long v, const Mask<W> &m) {
   *reinterpret_cast<Mask<W> *>(base + v * W) = m;
@@ -541,8 +560,8 @@ __device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, lon
 // remaining lists are scanned by the whole wavefront, one vertex at a time,
 // with a wave OR per 64 neighbours. When this level also pushed some words
 // (a top-down launch ran just before), the bits pushed to v are merged in.
-template <int BS, int W>
-__device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
+template <int BS, int W, int H>
+__device__ void bbu_phase_h(const BArgs &a, BSmem<BS> &sm, int L) {
   const unsigned long long *fs = level_ptr(a, L);
   const unsigned long long *jf = a.jf[L & 1];  // joint frontier of level L
   unsigned long long *nx = level_ptr(a, L + 1);
@@ -591,8 +610,8 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
         Mask<W> f0, f1;
 #pragma unroll
         for (int w = 0; w < W; ++w) f0.w[w] = f1.w[w] = 0;
-        if (in0) f0 = load_mask<W>(fs, u0);
-        if (in1) f1 = load_mask<W>(fs, u1);
+        if (in0) f0 = gather_mask<W, H>(fs, u0);
+        if (in1) f1 = gather_mask<W, H>(fs, u1);
         bool all = true;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -628,7 +647,7 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
         for (int w = 0; w < W; ++w) f.w[w] = 0;
         if (q < ee) {
           const int u = a.adj[q];
-          if (bitv(jf, u)) f = load_mask<W>(fs, u);
+          if (bitv(jf, u)) f = gather_mask<W, H>(fs, u);
         }
         bool all = true;
 #pragma unroll
@@ -662,6 +681,16 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     if (lane == 0) jn[wd] = bits;
   }
   wave_counts<BS, W>(sm, jw, mf, alg, munf, act, 0, 0, mfw);
+}
+
+// Bottom-up level L, gathering one half of each neighbour's mask when every
+// pull word lies in it (W = 8).
+template <int BS, int W>
+__device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
+  const unsigned pull = sm.S.pull_words;
+  if (W == 8 && (pull & 0xF0u) == 0) bbu_phase_h<BS, W, 0>(a, sm, L);
+  else if (W == 8 && (pull & 0x0Fu) == 0) bbu_phase_h<BS, W, 1>(a, sm, L);
+  else bbu_phase_h<BS, W, -1>(a, sm, L);
 }
 
 // Output: per 64-vertex segment, replay the level masks of each 64-source
