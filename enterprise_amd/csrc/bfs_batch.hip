@@ -7,8 +7,15 @@
 // is source s):
 //   seen  u64[n * W]              sources that reached v
 //   lv    u64[(lcap + 3) * n * W] lv[L][v] = sources whose level-L frontier
-//                                 holds v (new bits only; kept for the output)
-//   jf[2] u64[ceil(n/64)]         joint frontier bitmaps (v in any frontier)
+//                                 holds v (new bits only; kept for the output);
+//                                 defined only where jf[L] has v's bit
+//   jf    u64[(lcap + 3) * ceil(n/64)]  jf[L] = joint frontier of level L
+//                                 (v in the level-L frontier of any source)
+// Nothing reads lv[L][v] unless v is in jf[L], so lv is never cleared as a
+// whole: a level clears lv[L + 2][v] only for the vertices that the next
+// level's push can reach (unfinished, with edges), and the two readers of a
+// row that may be stale (APPLY, and the pull's merge of pushed bits) read it
+// only for such vertices.
 // One launch index per phase, with the same launch-indexed state scheme as
 // the single-source path (launch i reads the state and finished counters of
 // launch i - 1; no fences, no contended atomics). The direction is chosen
@@ -58,7 +65,8 @@ enum : int {
   C_EDGES = 12,
   C_VERTS = 13,
   C_MFW = 14,
-  C_COUNT = 22
+  C_WORK = 22,  // output: next 64-vertex segment to claim (not a count)
+  C_COUNT = 23
 };
 
 struct BCounters {
@@ -96,7 +104,7 @@ struct BArgs {
   const int *piece_k;
   unsigned long long *seen;
   unsigned long long *lv;
-  unsigned long long *jf[2];
+  unsigned long long *jf;  // lcap + 3 joint-frontier bitmaps of nwords
   int *out;
   BControl *ctl;
   long long m;
@@ -148,6 +156,16 @@ __device__ __forceinline__ void store_mask(unsigned long long *base, long long v
 }
 
 __device__ __forceinline__ unsigned long long *level_ptr(const BArgs &a, int L) { return a.lv + (long long)L * a.nW; }
+__device__ __forceinline__ unsigned long long *jf_ptr(const BArgs &a, int L) {
+  return a.jf + (long long)L * a.nwords;
+}
+
+// True if a push can reach v. On a symmetric graph (the only kind the pull
+// searches exactly) a vertex without edges is no one's neighbour; a forced
+// top-down search, exact on any adjacency, assumes every vertex can be.
+__device__ __forceinline__ bool receives(const BArgs &a, long long deg) {
+  return deg > 0 || a.mode == ENT_DIR_TOPDOWN;
+}
 
 template <int BS>
 struct BSmem {
@@ -156,6 +174,7 @@ struct BSmem {
   int excl[NW][kWave];
   unsigned char d8[NW][kWave * 68];  // output staging: 64 lanes x 64 sources (+4 pad)
   unsigned long long c[C_COUNT];
+  long long seg;  // output: the segment the block claimed
   BState S;
 };
 
@@ -377,7 +396,7 @@ template <int BS, int W>
 __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const unsigned long long *fs = level_ptr(a, L);
   unsigned long long *dst = level_ptr(a, L + 1);
-  const unsigned long long *jf = a.jf[L & 1];
+  const unsigned long long *jf = jf_ptr(a, L);
   const unsigned pw = sm.S.push_words;  // only these words are pushed at this level
   for (int p = blockIdx.x; p < a.n_pieces; p += gridDim.x) {
     const int v = a.piece_v[p];
@@ -444,8 +463,9 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   }
 }
 
-// Shared tail of APPLY and BU for the vertex of this lane: new bits x (the
-// lane's vertex v < n when valid), seen update, next-level zeroing, counts.
+// Tail of APPLY for the vertex of this lane: new bits x (the lane's vertex
+// v < n when valid), seen update, counts, and the clearing of lv[L + 2][v]
+// when the next level's push can still reach v.
 template <int W>
 __device__ __forceinline__ void settle_vertex(const BArgs &a, int L, bool valid, long long v, const Mask<W> &sd,
                                               const Mask<W> &x, long long deg, const unsigned long long *active,
@@ -474,16 +494,20 @@ __device__ __forceinline__ void settle_vertex(const BArgs &a, int L, bool valid,
 #pragma unroll
   for (int w = 0; w < W; ++w) unf |= (active[w] & ~ns.w[w]) != 0;
   if (unf) munf += (unsigned long long)deg;
-  Mask<W> z;
+  // The next level pushes only bits of sources active now, and only to
+  // vertices still missing one of them.
+  if (unf && receives(a, deg)) {
+    Mask<W> z;
 #pragma unroll
-  for (int w = 0; w < W; ++w) z.w[w] = 0;
-  store_mask<W>(level_ptr(a, L + 2), v, z);
+    for (int w = 0; w < W; ++w) z.w[w] = 0;
+    store_mask<W>(level_ptr(a, L + 2), v, z);
+  }
 }
 
 template <int BS, int W>
 __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   unsigned long long *nx = level_ptr(a, L + 1);
-  unsigned long long *jn = a.jf[(L + 1) & 1];
+  unsigned long long *jn = jf_ptr(a, L + 1);
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
@@ -499,9 +523,14 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     for (int w = 0; w < W; ++w) x.w[w] = sd.w[w] = 0;
     long long deg = 0;
     if (valid) {
-      x = load_mask<W>(nx, v);
       sd = load_mask<W>(a.seen, v);
       deg = a.off[v + 1] - a.off[v];
+      // lv[L + 1][v] was cleared (then pushed to) iff v still missed an
+      // active source and can receive pushes; otherwise nothing reached it.
+      bool open = false;
+#pragma unroll
+      for (int w = 0; w < W; ++w) open |= (sm.S.active[w] & ~sd.w[w]) != 0;
+      if (open && receives(a, deg)) x = load_mask<W>(nx, v);
     }
     bool any;
     settle_vertex<W>(a, L, valid, v, sd, x, deg, sm.S.active, jw, mf, alg, munf, act, mfw, any);
@@ -546,11 +575,15 @@ __device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, lon
     mf += (unsigned long long)deg;
     alg += (unsigned long long)nzw * (8ull + 12ull * (unsigned long long)deg);
   }
-  if (xc < ftc) munf += (unsigned long long)deg;
-  Mask<W> z;
+  if (xc < ftc) {  // unfinished: the next level's push can still reach v
+    munf += (unsigned long long)deg;
+    if (receives(a, deg)) {
+      Mask<W> z;
 #pragma unroll
-  for (int w = 0; w < W; ++w) z.w[w] = 0;
-  store_mask<W>(level_ptr(a, L + 2), v, z);
+      for (int w = 0; w < W; ++w) z.w[w] = 0;
+      store_mask<W>(level_ptr(a, L + 2), v, z);
+    }
+  }
 }
 
 // Bottom-up (pull) level: every vertex v gathers the level-L masks of its
@@ -563,9 +596,9 @@ __device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, lon
 template <int BS, int W, int H>
 __device__ void bbu_phase_h(const BArgs &a, BSmem<BS> &sm, int L) {
   const unsigned long long *fs = level_ptr(a, L);
-  const unsigned long long *jf = a.jf[L & 1];  // joint frontier of level L
+  const unsigned long long *jf = jf_ptr(a, L);  // joint frontier of level L
   unsigned long long *nx = level_ptr(a, L + 1);
-  unsigned long long *jn = a.jf[(L + 1) & 1];
+  unsigned long long *jn = jf_ptr(a, L + 1);
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
@@ -662,7 +695,9 @@ __device__ void bbu_phase_h(const BArgs &a, BSmem<BS> &sm, int L) {
     Mask<W> x;
 #pragma unroll
     for (int w = 0; w < W; ++w) x.w[w] = acc.w[w] & tgt.w[w];
-    if (valid && merge) {  // bits pushed to v by this level's top-down launch
+    // Bits pushed to v by this level's top-down launch: lv[L + 1][v] was
+    // cleared for v iff v missed an active source (ftc > 0) and has edges.
+    if (valid && merge && ftc > 0 && receives(a, e - s)) {
       const Mask<W> xp = load_mask<W>(nx, v);
 #pragma unroll
       for (int w = 0; w < W; ++w) x.w[w] |= xp.w[w];
@@ -695,11 +730,16 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
 
 // Output: per 64-vertex segment, replay the level masks of each 64-source
 // group into an LDS table of levels (u8, 0xFF = unreached) and store the 64
-// rows of the group with 256-byte coalesced writes. The wavefronts of a block
-// share one segment (wavefront w takes groups w, w + 4, ...), so the 64-byte
-// mask line of a vertex is fetched once and read by every group while hot.
+// rows of the group with 256-byte coalesced writes. A vertex's mask line of
+// level L is fetched only when the vertex is in that level's joint frontier
+// (about 2 of 8 levels at R-MAT scale 23, none for a vertex without edges).
+// The wavefronts of a block share one segment (wavefront w takes groups w,
+// w + 4, ...), so each fetched line is read by every group while hot.
+// Blocks claim segments from a counter of the launch (work): the grid holds
+// more blocks than are resident at once, and blocks that start late take
+// fewer segments instead of an equal share.
 template <int BS, int W>
-__device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
+__device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels, unsigned long long *work) {
   constexpr int NWV = BS / kWave;
   const int wid = threadIdx.x / kWave, lane = lane_id();
   unsigned char *d8 = sm.d8[wid];
@@ -710,7 +750,12 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
   Mask<W> cm;
 #pragma unroll
   for (int w = 0; w < W; ++w) cm.w[w] = trunc ? ~sm.S.active[w] : ~0ull;
-  for (long long wd = blockIdx.x; wd < a.nwords; wd += gridDim.x) {
+  for (;;) {
+    if (threadIdx.x == 0) sm.seg = (long long)atomicAdd(work, 1ull);
+    __syncthreads();
+    const long long wd = sm.seg;
+    __syncthreads();
+    if (wd >= a.nwords) break;
     const long long v = wd * 64 + lane;
     const bool valid = v < a.n;
     if (valid && wid == 0) {
@@ -725,13 +770,21 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels) {
       unsigned int *row = reinterpret_cast<unsigned int *>(d8 + lane * 68);
 #pragma unroll
       for (int j = 0; j < 16; ++j) row[j] = 0xFFFFFFFFu;
-      if (valid) {
-        for (int L = 0; L < levels; ++L) {
-          unsigned long long mk = level_ptr(a, L)[v * W + g];
-          while (mk) {
-            const int b = __ffsll((long long)mk) - 1;
-            mk &= mk - 1;
-            d8[lane * 68 + b] = (unsigned char)L;
+      for (int L0 = 0; L0 < levels; L0 += kWave) {
+        // Lane j holds the segment's joint-frontier word of level L0 + j.
+        const unsigned long long fl = L0 + lane < levels ? jf_ptr(a, L0 + lane)[wd] : 0ull;
+        unsigned long long lm = __ballot(fl != 0);
+        while (lm) {
+          const int j = __ffsll((long long)lm) - 1;
+          lm &= lm - 1;
+          const unsigned long long fw = read_lane64(fl, j);
+          if (((fw >> lane) & 1ull) != 0) {
+            unsigned long long mk = level_ptr(a, L0 + j)[v * W + g];
+            while (mk) {
+              const int b = __ffsll((long long)mk) - 1;
+              mk &= mk - 1;
+              d8[lane * 68 + b] = (unsigned char)(L0 + j);
+            }
           }
         }
       }
@@ -787,7 +840,7 @@ __global__ __launch_bounds__(BS) void k_blevel(BArgs a, long long i) {
   switch (S.phase) {
     case BPH_TD: btd_phase<BS, W>(a, sm, S.level); break;
     case BPH_APPLY: bapply_phase<BS, W>(a, sm, S.level); break;
-    default: boutput_phase<BS, W>(a, sm, S.levels); break;
+    default: boutput_phase<BS, W>(a, sm, S.levels, &a.ctl->cnt[slot].v[C_WORK]); break;
   }
   if (S.phase != BPH_TD) block_counts_flush(a, sm, slot);
 }
@@ -802,8 +855,6 @@ __global__ __launch_bounds__(BS) void k_blevel_bu(BArgs a, long long i) {
   block_counts_flush(a, sm, int(i % 3));
 }
 
-// Zeroes seen, lv[0], lv[1] and the joint frontier; clears counters; sets the
-// pseudo-level -1 state.
 // The whole multi-source search in one cooperative launch: launches 0, 1,
 // 2, ... of the per-phase form separated by grid barriers. Stops on
 // completion, or when a barrier wait exceeds ~4 s in any block (the search is
@@ -828,7 +879,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(bpersist_wav
       case BPH_TD: btd_phase<BS, W>(a, sm, S.level); break;
       case BPH_APPLY: bapply_phase<BS, W>(a, sm, S.level); break;
       case BPH_BU: bbu_phase<BS, W>(a, sm, S.level); break;
-      default: boutput_phase<BS, W>(a, sm, S.levels); break;
+      default: boutput_phase<BS, W>(a, sm, S.levels, &a.ctl->cnt[slot].v[C_WORK]); break;
     }
     if (S.phase != BPH_TD) block_counts_flush(a, sm, slot);
     const bool ok = grid_sync(&a.ctl->sync, 400000000ull);
@@ -844,19 +895,22 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(bpersist_wav
   }
 }
 
-__global__ void k_binit_zero(BArgs a) {
+// Zeroes seen, the level-0 joint frontier, lv[0] at the roots (the only
+// vertices where it is read) and lv[1] where level 0 can push; clears the
+// counters and the grid barrier; sets the pseudo-level -1 state.
+__global__ void k_binit_zero(BArgs a, const int *roots) {
   const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long nt = (long long)gridDim.x * blockDim.x;
-  unsigned long long *l0 = a.lv, *l1 = a.lv + a.nW;
+  const int W = int(a.nW / a.n);
+  const int sh = W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : 3;
+  unsigned long long *l1 = a.lv + a.nW;
   for (long long i = gt; i < a.nW; i += nt) {
     a.seen[i] = 0;
-    l0[i] = 0;
-    l1[i] = 0;
+    const long long v = i >> sh;
+    if (receives(a, a.off[v + 1] - a.off[v])) l1[i] = 0;
   }
-  for (long long w = gt; w < a.nwords; w += nt) {
-    a.jf[0][w] = 0;
-    a.jf[1][w] = 0;
-  }
+  for (long long i = gt; i < (long long)a.k * W; i += nt) a.lv[(long long)roots[i / W] * W + i % W] = 0;
+  for (long long w = gt; w < a.nwords; w += nt) a.jf[w] = 0;
   constexpr long long kC = (long long)kBStripes * kBStripeU64;
   for (long long i = gt; i < 3 * kC; i += nt) a.ctl->cnt[i / kC].v[i % kC] = 0;
   for (long long i = gt; i < kSyncStripes * kSyncStride; i += nt) a.ctl->sync.arrive[i] = 0;
@@ -888,7 +942,7 @@ __global__ void k_binit_src(BArgs a, const int *roots, BInit in) {
     const unsigned long long bit = 1ull << (s & 63);
     atomicOr(&a.lv[(long long)v * W + (s >> 6)], bit);
     atomicOr(&a.seen[(long long)v * W + (s >> 6)], bit);
-    atomicOr(&a.jf[0][v >> 6], 1ull << (v & 63));
+    atomicOr(&a.jf[v >> 6], 1ull << (v & 63));
     atomicOr(&a.ctl->cnt[2].v[C_ACTIVE + (s >> 6)], bit);
   }
   if (s == 0) {
@@ -943,7 +997,7 @@ struct BatchWorkspace {
   int64_t n = 0;
   unsigned long long *seen = nullptr;
   unsigned long long *lv = nullptr;
-  unsigned long long *jf[2] = {nullptr, nullptr};
+  unsigned long long *jf = nullptr;  // lcap + 3 bitmaps
   int *roots = nullptr;
   BControl *ctl = nullptr;
   BControl *h_ctl = nullptr;
@@ -967,7 +1021,7 @@ int words_for(int k) { return k <= 64 ? 1 : k <= 128 ? 2 : k <= 256 ? 4 : 8; }
 size_t ws_bytes(int64_t n, int W, int lcap) {
   const size_t nW = size_t(n) * size_t(W);
   const size_t nwords = size_t((n + 63) / 64);
-  return 8 * nW + 8 * nW * size_t(lcap + 3) + 16 * nwords + sizeof(BControl) + 4 * ENT_BATCH_MAX_SOURCES;
+  return 8 * nW + (8 * nW + 8 * nwords) * size_t(lcap + 3) + sizeof(BControl) + 4 * ENT_BATCH_MAX_SOURCES;
 }
 
 int reserve(ent_graph *g, int k, int lcap) {
@@ -990,8 +1044,7 @@ int reserve(ent_graph *g, int k, int lcap) {
   g->batch = b;
   if ((rc = dev_alloc(g, reinterpret_cast<void **>(&b->seen), 8 * nW)) ||
       (rc = dev_alloc(g, reinterpret_cast<void **>(&b->lv), 8 * nW * size_t(lcap + 3))) ||
-      (rc = dev_alloc(g, reinterpret_cast<void **>(&b->jf[0]), 8 * size_t(g->nwords))) ||
-      (rc = dev_alloc(g, reinterpret_cast<void **>(&b->jf[1]), 8 * size_t(g->nwords))) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&b->jf), 8 * size_t(g->nwords) * size_t(lcap + 3))) ||
       (rc = dev_alloc(g, reinterpret_cast<void **>(&b->roots), 4 * ENT_BATCH_MAX_SOURCES)) ||
       (rc = dev_alloc(g, reinterpret_cast<void **>(&b->ctl), sizeof(BControl)))) {
     batch_release(g);
@@ -1036,8 +1089,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   a.piece_k = g->piece_k;
   a.seen = b->seen;
   a.lv = b->lv;
-  a.jf[0] = b->jf[0];
-  a.jf[1] = b->jf[1];
+  a.jf = b->jf;
   a.out = out;
   a.ctl = b->ctl;
   a.m = g->m;
@@ -1080,7 +1132,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   HIP_TRY(hipMemcpyAsync(b->roots, roots, sizeof(int32_t) * size_t(k), hipMemcpyHostToDevice, g->stream));
   const int grid = g->cus * 8;
   {
-    void *args[] = {&a};
+    void *args[] = {&a, &b->roots};
     HIP_TRY(hipLaunchKernel(reinterpret_cast<const void *>(&k_binit_zero), dim3(grid), dim3(256), args, 0, g->stream));
   }
   {
@@ -1143,8 +1195,7 @@ void batch_release(ent_graph *g) {
   const size_t nW = size_t(b->n) * size_t(b->words);
   dev_free(g, b->seen, 8 * nW);
   dev_free(g, b->lv, 8 * nW * size_t(b->lcap + 3));
-  dev_free(g, b->jf[0], 8 * size_t(g->nwords));
-  dev_free(g, b->jf[1], 8 * size_t(g->nwords));
+  dev_free(g, b->jf, 8 * size_t(g->nwords) * size_t(b->lcap + 3));
   dev_free(g, b->roots, 4 * ENT_BATCH_MAX_SOURCES);
   dev_free(g, b->ctl, sizeof(BControl));
   dev_free(g, b->rows, b->rows_bytes);

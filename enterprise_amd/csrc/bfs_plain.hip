@@ -952,7 +952,10 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
 }
 
 // depth = -1 (root 0), visited = isolated vertices, padding bits and the root,
-// front[0] = {root}, marks cleared. The state before launch 0 is a finished
+// front[0] = {root}, marks cleared. A vertex without edges is no one's
+// neighbour on a symmetric adjacency, the only kind bottom-up levels search
+// exactly; a forced top-down search, exact on any adjacency, marks only the
+// padding bits, since a vertex without out-edges may still have in-edges. The state before launch 0 is a finished
 // pseudo-level -1 whose counts (cnt[2]) describe frontier 0 = {root}.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu, int search) {
@@ -980,7 +983,8 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu, int
   if (gt < (long long)a.n - (nm << 4)) a.mark[(nm << 4) + gt] = 0;
   for (long long w = gt; w < a.nwords; w += nt) {
     const unsigned long long rb = (w == (root >> 6)) ? 1ull << (root & 63) : 0ull;
-    a.visited[w] = ~a.nzmask[w] | rb;
+    const unsigned long long pad = (w == a.nwords - 1 && (a.n & 63)) ? ~0ull << (a.n & 63) : 0ull;
+    a.visited[w] = (a.mode == ENT_DIR_TOPDOWN ? pad : ~a.nzmask[w]) | rb;
     a.front[0][w] = rb;
     a.front[1][w] = 0ull;
   }

@@ -8,9 +8,10 @@
  * whose frontier touches it. The result is k int32 depth rows, row-major
  * (depths[s * n + v]), each identical to what ent_bfs gives for roots[s].
  *
- * Memory: besides the graph, a workspace of (max_levels + 3) * n * W * 8
- * bytes holds the per-level frontier masks (levels are replayed at the end
- * to write every output row in full, coalesced lines). A search that needs
+ * Memory: besides the graph, a workspace of (max_levels + 3) * (n * W * 8 +
+ * ceil(n / 64) * 8) bytes holds the per-level frontier masks and joint
+ * frontier bitmaps (levels are replayed at the end to write every output
+ * row in full, coalesced lines). A search that needs
  * more joint levels than reserved still returns every row complete: the
  * rows of sources that finished within the reserved levels come from the
  * multi-source search, and each source still unfinished is searched again on

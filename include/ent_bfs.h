@@ -32,7 +32,10 @@ extern "C" {
 
 typedef struct ent_graph ent_graph;
 
-/* Direction policy. */
+/* Direction policy. Bottom-up levels scan a vertex's own adjacency for a
+ * parent, as the reference's do, so every policy but ENT_DIR_TOPDOWN needs
+ * a symmetric adjacency (each edge stored both ways); ENT_DIR_TOPDOWN is
+ * exact on any adjacency. */
 #define ENT_DIR_AUTO 0      /* Beamer alpha/beta switch, both ways */
 #define ENT_DIR_REFERENCE 1 /* reference rule: bottom-up once the frontier's
                                degree sum reaches 0.3 * m at a level >= 1,
@@ -146,8 +149,9 @@ int ent_graph_free(ent_graph *g);
  * unreached vertices. The result is a BFS tree that passes the Graph500
  * parent checks. Returns ENT_EINVAL (parents of the offending vertices set to
  * -2) when some reached vertex has no neighbour one level closer, i.e. the
- * depths are not a BFS from root. No reference counterpart: the reference
- * outputs depths only (bfs_gpu_opt.cuh:446). */
+ * depths are not a BFS from root. Needs a symmetric adjacency (a vertex's
+ * neighbours are then its in-neighbours). No reference counterpart: the
+ * reference outputs depths only (bfs_gpu_opt.cuh:446). */
 int ent_bfs_parents_device(ent_graph *g, int32_t root, const int32_t *depth_dev,
                            int32_t *parent_dev);
 
@@ -161,7 +165,7 @@ int ent_bfs_parents_device(ent_graph *g, int32_t root, const int32_t *depth_dev,
  *   counts[3] = reached vertices other than root with no neighbour one level
  *               closer.
  * All four are zero exactly when depth is the BFS from root (hop distances
- * are unique). Returns ENT_OK whatever the counts. */
+ * are unique) on a symmetric adjacency. Returns ENT_OK whatever the counts. */
 int ent_bfs_validate_device(ent_graph *g, int32_t root, const int32_t *depth_dev,
                             int64_t counts[4]);
 

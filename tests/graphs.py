@@ -101,6 +101,23 @@ def disjoint_paths(n1, n2):
     return edges_graph(n, edges), depth
 
 
+def directed_with_sinks(n_src, n_sink, deg, seed):
+    """Directed CSR: vertices 0..n_src-1 each have `deg` distinct out-edges
+    to random targets over all vertices (no loops); the n_sink vertices after
+    them have in-edges only."""
+    rng = np.random.default_rng(seed)
+    n = n_src + n_sink
+    rows = []
+    for v in range(n_src):
+        t = rng.choice(n - 1, size=deg, replace=False)
+        t[t >= v] += 1
+        rows.append(np.sort(t).astype(np.int32))
+    beg = np.zeros(n + 1, dtype=np.int64)
+    beg[1:n_src + 1] = np.cumsum([len(r) for r in rows])
+    beg[n_src + 1:] = beg[n_src]
+    return beg, np.concatenate(rows)
+
+
 CLOSED_FORMS = {
     "path300": lambda: path(300),
     "cycle257": lambda: cycle(257),

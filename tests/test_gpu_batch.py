@@ -225,3 +225,47 @@ def test_batch_split_levels(oracle, rmat16):
             assert not (st["bu_level_mask"] & st["td_level_mask"]), st
     finally:
         dg.set_config(base)
+
+
+def test_batch_stale_level_rows(oracle, rmat16):
+    """Level masks are defined only where the level's joint frontier holds
+    the vertex, and a level clears the next-but-one level's masks only for
+    vertices the next push can reach. Batches that leave different masks
+    behind run back to back on one workspace (hub roots, then low-degree
+    roots, then the hub roots again, in both launch forms), and every row
+    still equals the single-source result."""
+    g, dg = rmat16
+    hubs = g.pick_roots_mixed(256, 256, 3031)
+    low = g.pick_roots_mixed(192, 0, 3032)
+    base = dg.config
+    try:
+        for launch in (L.LAUNCH_STREAM, L.LAUNCH_PERSISTENT):
+            dg.configure(launch=launch)
+            for j, roots in enumerate((hubs, low, hubs)):
+                _check_batch(oracle, g, dg, roots, f"rmat16/stale/{launch}/{j}")
+    finally:
+        dg.set_config(base)
+
+
+def test_batch_directed_topdown(oracle):
+    """A forced top-down batch is exact on a directed adjacency: vertices
+    without out-edges are still reached through their in-edges (their level
+    masks are cleared like every other unfinished vertex's). Two batches in
+    a row on one workspace, against the oracle and the single-source path."""
+    beg, adj = graphs.directed_with_sinks(3000, 1000, 3, 77)
+    n = beg.size - 1
+    rng = np.random.default_rng(78)
+    with DeviceGraph(beg_pos=beg, adj=adj) as dg:
+        dg.configure(direction=L.DIR_TOPDOWN)
+        for j, k in enumerate((130, 64)):
+            roots = rng.choice(3000, size=k, replace=False).astype(np.int32)
+            _say(f"[gpu] batch directed/topdown k={k}")
+            out, st = dg.bfs_batch(roots)
+            for i, r in enumerate(roots):
+                want = oracle.bfs(beg, adj, int(r))
+                assert np.array_equal(out[i], want), (j, i, int(r))
+                single, _ = dg.bfs(int(r))
+                assert np.array_equal(single, want), (j, i, int(r))
+            assert np.any(out[:, 3000:] > 0)                 # sinks reached
+            assert st["levels"] == int(out.max()) + 1
+            assert n == 4000

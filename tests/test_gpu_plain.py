@@ -371,3 +371,22 @@ def test_unaligned_depth_arrays(oracle, n):
                 assert (got[:k] == guard).all() and (got[k + n:] == guard).all(), (n, k)
         finally:
             dg.device_free(base)
+
+
+@pytest.mark.parametrize("launch", list(LAUNCHES))
+def test_topdown_directed_sinks(oracle, launch):
+    """Forced top-down is exact on a directed adjacency: the 1,000 vertices
+    without out-edges are reached through their in-edges (they are not
+    taken as isolated), in every launch form, with tiny class thresholds."""
+    beg, adj = graphs.directed_with_sinks(3000, 1000, 3, 77)
+    with DeviceGraph(beg_pos=beg, adj=adj) as dg:
+        dg.configure(direction=L.DIR_TOPDOWN, launch=LAUNCHES[launch], **TINY_CLASSES)
+        for r in (0, 1234, 2999):
+            _say(f"[gpu] directed/topdown {launch} root {r}")
+            got, st = dg.bfs(r)
+            want = oracle.bfs(beg, adj, r)
+            assert np.array_equal(got, want), (launch, r)
+            assert np.any(want[3000:] > 0)
+            reached = want >= 0
+            assert st["reached_vertices"] == int(reached.sum())
+            assert st["reached_edges"] == int(np.diff(beg)[reached].sum())

@@ -47,6 +47,22 @@ def test_random_graphs_vs_scipy(oracle, seed):
         assert e == int(np.diff(g.beg_pos)[want >= 0].sum())
 
 
+def test_directed_vs_scipy(oracle):
+    """The oracle's top-down BFS follows out-edges only, as scipy's directed
+    shortest paths do: on a directed adjacency with 1,000 vertices that have
+    in-edges only, every depth matches (the GPU forced top-down tests check
+    against it there)."""
+    beg, adj = graphs.directed_with_sinks(3000, 1000, 3, 77)
+    n = beg.size - 1
+    a = sp.csr_matrix((np.ones(adj.size), adj, beg), shape=(n, n))
+    for r in (0, 1234, 2999):
+        d = shortest_path(a, directed=True, unweighted=True, indices=r)
+        want = np.where(np.isinf(d), -1, d).astype(np.int32)
+        got = oracle.bfs(beg, adj, r)
+        assert np.array_equal(got, want), r
+        assert np.any(want[3000:] > 0) and np.all(np.diff(beg)[3000:] == 0)
+
+
 def test_golden_vectors(oracle):
     data = np.load(os.path.join(GOLDEN, "small_graphs.npz"))
     names = sorted({k.split("__")[0] for k in data.files})
