@@ -523,14 +523,20 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     for (int w = 0; w < W; ++w) x.w[w] = sd.w[w] = 0;
     long long deg = 0;
     if (valid) {
+      x = load_mask<W>(nx, v);
       sd = load_mask<W>(a.seen, v);
       deg = a.off[v + 1] - a.off[v];
       // lv[L + 1][v] was cleared (then pushed to) iff v still missed an
-      // active source and can receive pushes; otherwise nothing reached it.
+      // active source and can receive pushes; otherwise it is stale and
+      // nothing reached v. (Loaded with seen, not after it: the dependent
+      // load costs the shared kernel a wavefront per SIMD in registers.)
       bool open = false;
 #pragma unroll
       for (int w = 0; w < W; ++w) open |= (sm.S.active[w] & ~sd.w[w]) != 0;
-      if (open && receives(a, deg)) x = load_mask<W>(nx, v);
+      if (!(open && receives(a, deg))) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) x.w[w] = 0;
+      }
     }
     bool any;
     settle_vertex<W>(a, L, valid, v, sd, x, deg, sm.S.active, jw, mf, alg, munf, act, mfw, any);
@@ -774,16 +780,36 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels, unsigne
         // Lane j holds the segment's joint-frontier word of level L0 + j.
         const unsigned long long fl = L0 + lane < levels ? jf_ptr(a, L0 + lane)[wd] : 0ull;
         unsigned long long lm = __ballot(fl != 0);
+        // Up to kLv levels per round trip: their mask loads are issued
+        // together, then replayed (a segment's wavefront visits about six
+        // levels at R-MAT scale 23, one lane's vertex about two).
         while (lm) {
-          const int j = __ffsll((long long)lm) - 1;
-          lm &= lm - 1;
-          const unsigned long long fw = read_lane64(fl, j);
-          if (((fw >> lane) & 1ull) != 0) {
-            unsigned long long mk = level_ptr(a, L0 + j)[v * W + g];
-            while (mk) {
-              const int b = __ffsll((long long)mk) - 1;
-              mk &= mk - 1;
-              d8[lane * 68 + b] = (unsigned char)(L0 + j);
+          constexpr int kLv = 4;
+          int jj[kLv];
+          unsigned long long mk[kLv];
+#pragma unroll
+          for (int t = 0; t < kLv; ++t) {
+            jj[t] = -1;
+            mk[t] = 0;
+            if (lm) {
+              jj[t] = __ffsll((long long)lm) - 1;
+              lm &= lm - 1;
+            }
+          }
+#pragma unroll
+          for (int t = 0; t < kLv; ++t) {
+            if (jj[t] >= 0) {
+              const unsigned long long fw = read_lane64(fl, jj[t]);
+              if (((fw >> lane) & 1ull) != 0) mk[t] = level_ptr(a, L0 + jj[t])[v * W + g];
+            }
+          }
+#pragma unroll
+          for (int t = 0; t < kLv; ++t) {
+            unsigned long long m = mk[t];
+            while (m) {
+              const int b = __ffsll((long long)m) - 1;
+              m &= m - 1;
+              d8[lane * 68 + b] = (unsigned char)(L0 + jj[t]);
             }
           }
         }
