@@ -734,6 +734,13 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   else bbu_phase_h<BS, W, -1>(a, sm, L);
 }
 
+// 0xFF in byte j of the result for each set bit j of bits 4k .. 4k+3 of m.
+__device__ __forceinline__ unsigned nibble_bytes(unsigned long long m, int k) {
+  unsigned x = (unsigned)(m >> (4 * k)) & 0xFu;
+  x = (x | (x << 7) | (x << 14) | (x << 21)) & 0x01010101u;
+  return x * 0xFFu;
+}
+
 // Output: per 64-vertex segment, replay the level masks of each 64-source
 // group into an LDS table of levels (u8, 0xFF = unreached) and store the 64
 // rows of the group with 256-byte coalesced writes. A vertex's mask line of
@@ -803,13 +810,24 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels, unsigne
               if (((fw >> lane) & 1ull) != 0) mk[t] = level_ptr(a, L0 + jj[t])[v * W + g];
             }
           }
+          // Each 4 bits of a mask become 4 table bytes at once (the bits
+          // of one source are set at exactly one level).
+          unsigned long long any = 0;
 #pragma unroll
-          for (int t = 0; t < kLv; ++t) {
-            unsigned long long m = mk[t];
-            while (m) {
-              const int b = __ffsll((long long)m) - 1;
-              m &= m - 1;
-              d8[lane * 68 + b] = (unsigned char)(L0 + jj[t]);
+          for (int t = 0; t < kLv; ++t) any |= mk[t];
+          if (any) {
+            unsigned lrep[kLv];
+#pragma unroll
+            for (int t = 0; t < kLv; ++t) lrep[t] = (unsigned)(L0 + jj[t]) * 0x01010101u;
+#pragma unroll 4
+            for (int k = 0; k < 16; ++k) {
+              unsigned w = row[k];
+#pragma unroll
+              for (int t = 0; t < kLv; ++t) {
+                const unsigned msk = nibble_bytes(mk[t], k);
+                w = (w & ~msk) | (lrep[t] & msk);
+              }
+              row[k] = w;
             }
           }
         }
