@@ -333,7 +333,7 @@ def bench_plain(args):
         b = run_extra(args, "batch", scale=23, steps=3, warmup=1)
         out["extra"]["batch"] = {k: b[k] for k in (
             "value", "unit", "batch_ms", "batch_device_ms", "plain_aggregate_gteps", "speedup_vs_plain",
-            "rows_mismatching_plain", "levels", "fallback_rows", "roofline", "cpu_baseline", "config")}
+            "rows_mismatching_plain", "levels", "flushes", "roofline", "cpu_baseline", "config")}
         log(f"[bench] batch: {out['extra']['batch']['speedup_vs_plain']}x the plain path")
         log("[bench] extra: 4096^2 grid, launch forms")
         gr = run_extra(args, "grid", grid_roots=0.25, set=[])

@@ -88,7 +88,7 @@ class BatchStats(C.Structure):
                 ("reached_edges", C.c_int64), ("reached_vertices", C.c_int64),
                 ("alg_bytes", C.c_int64), ("device_ms", C.c_double),
                 ("host_ms", C.c_double), ("bu_level_mask", C.c_uint64),
-                ("td_level_mask", C.c_uint64), ("fallback_rows", C.c_int32), ("pad", C.c_int32)]
+                ("td_level_mask", C.c_uint64), ("flushes", C.c_int32), ("pad", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}

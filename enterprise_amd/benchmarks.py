@@ -113,7 +113,7 @@ def batch(args):
                      "frac": round(achieved / 8000.0, 4), "alg_bytes": best["alg_bytes"],
                      "traffic": tr, "traffic_over_alg": round(tr / best["alg_bytes"], 3) if tr else None},
         "runs_ms": [round(s["host_ms"], 3) for s in stats],
-        "fallback_rows": best["fallback_rows"],
+        "flushes": best["flushes"],
         # The graph and roots, for the CPU baseline that bench.py runs.
         "_cpu_batch": {"graph": g, "roots": roots},
     }
@@ -248,7 +248,7 @@ def serve(args):
         for rate in rates:
             for lane in srv.lanes.values():
                 lane.stats.update({"batches": 0, "plain_batches": 0, "queries": 0, "reached_edges": 0,
-                                   "alg_bytes": 0, "busy_s": 0.0, "batch_sizes": [], "fallback_rows": 0})
+                                   "alg_bytes": 0, "busy_s": 0.0, "batch_sizes": [], "flushes": 0})
             t1 = time.perf_counter()
             res = run_load(srv, rate, dur, 1005 + rate, roots, {"large": 0.8, "small": 0.2})
             wall = time.perf_counter() - t1
@@ -270,7 +270,7 @@ def serve(args):
                    "throughput_qps": round(len(res) / wall, 1), "gteps": round(edges / wall / 1e9, 2),
                    "alg_GBps": round(alg / wall / 1e9, 1),
                    "mean_batch": round(float(np.mean(sizes)), 1) if sizes else 0,
-                   "fallback_rows": sum(l.stats["fallback_rows"] for l in srv.lanes.values()),
+                   "flushes": sum(l.stats["flushes"] for l in srv.lanes.values()),
                    "checked": int(len(sample)), "mismatches": mism}
             log(f"[serve] {row}")
             per_rate.append(row)

@@ -6,11 +6,15 @@
 // Layout (vertex-major masks, W u64 words per vertex, bit s%64 of word s/64
 // is source s):
 //   seen  u64[n * W]              sources that reached v
-//   lv    u64[(lcap + 3) * n * W] lv[L][v] = sources whose level-L frontier
+//   lv    u64[R * n * W]          lv[L][v] = sources whose level-L frontier
 //                                 holds v (new bits only; kept for the output);
 //                                 defined only where jf[L] has v's bit
-//   jf    u64[(lcap + 3) * ceil(n/64)]  jf[L] = joint frontier of level L
+//   jf    u64[R * ceil(n/64)]     jf[L] = joint frontier of level L
 //                                 (v in the level-L frontier of any source)
+// Both are rings of R = lcap + 3 level slots (level L in slot L mod R). Before
+// a level would reuse the slot of a level not yet written to the output rows,
+// a flush pass writes the levels kept so far into the rows, so a search of
+// any depth runs as one batch in a fixed workspace.
 // Nothing reads lv[L][v] unless v is in jf[L], so lv is never cleared as a
 // whole: a level clears lv[L + 2][v] only for the vertices that the next
 // level's push can reach (unfinished, with edges), and the two readers of a
@@ -30,8 +34,9 @@
 //          sources ORs the level-L masks of its neighbours, stopping once
 //          every such source has been found (or the list ends); the bits
 //          pushed to it at this level are merged in;
-//   OUTPUT replays lv[0..levels) per 64-vertex segment in LDS and writes all
-//          k depth rows with 16-byte streaming stores.
+//   OUTPUT replays the kept levels per 64-vertex segment in LDS and writes
+//          all k depth rows with 16-byte streaming stores (a flush: the same
+//          for the levels kept so far, in the middle of the search).
 // The pull runs in its own kernel, launched beside the others for the same
 // launch index, so its mask registers do not cap their occupancy; with
 // ENT_LAUNCH_PERSISTENT the whole search is one cooperative launch instead.
@@ -50,7 +55,7 @@
 namespace ent {
 
 enum BPhase : int { BPH_TD = 0, BPH_BU = 1, BPH_APPLY = 2, BPH_OUTPUT = 3 };
-enum : int { BERR_LEVELS = 8, BERR_TIMEOUT = 16 };
+enum : int { BERR_TIMEOUT = 16 };
 
 constexpr int kBStripes = 64;
 constexpr int kBStripeU64 = 32;
@@ -88,6 +93,15 @@ struct BState {
   unsigned long long total_edges;
   unsigned long long total_vertices;
   unsigned long long active[8];
+  // Level ring (kept after the fields above: placed before pull_words, they
+  // changed the pull kernel's code generation, 116 instead of 122 VGPRs and
+  // 0.5 ms slower per R-MAT 23 batch).
+  int base;       // first level not yet written to the output rows
+  int flush;      // OUTPUT phase is a flush of levels base .. flush_end - 1
+  int flush_end;
+  int resume;     // phase to run after the flush
+  int flushes;    // flushes so far
+  int slot;       // ring slot of `level` in lv and jf
 };
 
 struct BControl {
@@ -104,7 +118,7 @@ struct BArgs {
   const int *piece_k;
   unsigned long long *seen;
   unsigned long long *lv;
-  unsigned long long *jf;  // lcap + 3 joint-frontier bitmaps of nwords
+  unsigned long long *jf;  // ring of lcap + 3 joint-frontier bitmaps of nwords
   int *out;
   BControl *ctl;
   long long m;
@@ -113,6 +127,7 @@ struct BArgs {
   int nwords;
   int k;
   int lcap;
+  int ring;  // lcap + 3 level slots in lv and jf
   int n_hub_block;
   int n_pieces;
   int t_wave, t_block, piece;
@@ -155,9 +170,17 @@ __device__ __forceinline__ void store_mask(unsigned long long *base, long long v
   *reinterpret_cast<Mask<W> *>(base + v * W) = m;
 }
 
-__device__ __forceinline__ unsigned long long *level_ptr(const BArgs &a, int L) { return a.lv + (long long)L * a.nW; }
-__device__ __forceinline__ unsigned long long *jf_ptr(const BArgs &a, int L) {
-  return a.jf + (long long)L * a.nwords;
+// Ring slot of level S.level + d, d = 0, 1 or 2 (the ring has at least four
+// slots; the state carries the slot of its level, so no division is needed).
+__device__ __forceinline__ int ring_slot(const BArgs &a, const BState &S, int d) {
+  const int t = S.slot + d;
+  return t >= a.ring ? t - a.ring : t;
+}
+__device__ __forceinline__ unsigned long long *level_ptr(const BArgs &a, int slot) {
+  return a.lv + (long long)slot * a.nW;
+}
+__device__ __forceinline__ unsigned long long *jf_ptr(const BArgs &a, int slot) {
+  return a.jf + (long long)slot * a.nwords;
 }
 
 // True if a push can reach v. On a symmetric graph (the only kind the pull
@@ -270,6 +293,12 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
     return s;
   }
   if (p.phase == BPH_OUTPUT) {
+    if (p.flush) {  // levels base .. flush_end - 1 are in the rows: resume
+      s.flush = 0;
+      s.base = p.flush_end;
+      s.phase = p.resume;
+      return s;
+    }
     s.total_edges = tot[C_EDGES];
     s.total_vertices = tot[C_VERTS];
     s.alg_bytes += 4ull * 64ull * (unsigned long long)W * (unsigned long long)a.n;
@@ -286,15 +315,6 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
     return s;
   }
   for (int w = 0; w < 8; ++w) s.active[w] = w < W ? tot[C_ACTIVE + w] : 0ull;
-  if (L + 1 > a.lcap) {
-    // Out of reserved levels: the output pass writes what levels 0..L found,
-    // and the sources still active (s.active) are left unfinished for the
-    // host to search one at a time.
-    s.error |= BERR_LEVELS;
-    s.levels = L + 1;
-    s.phase = BPH_OUTPUT;
-    return s;
-  }
   const unsigned long long mf = tot[C_MF], munf = tot[C_MUNF];
   // Direction per 64-source word: a word whose new frontier is large is
   // pulled, a word that lags is pushed, so a pull can stop once the large
@@ -315,9 +335,22 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
     if (bu) pull |= 1u << w;
   }
   s.level = L + 1;
+  s.slot = p.slot + 1 == a.ring ? 0 : p.slot + 1;
   s.pull_words = pull;
   s.push_words = live & ~pull;
   s.phase = s.push_words ? BPH_TD : BPH_BU;
+  // Level L' = s.level uses the slots of L' (its frontier), L' + 1 (its new
+  // bits) and L' + 2 (cleared for the next push). The ring holds them beside
+  // the levels base .. L' - 1 not yet in the rows while L' - base <= lcap;
+  // beyond that, levels base .. L' (L' complete: it is the new frontier) are
+  // flushed to the rows first.
+  if (s.level - s.base > a.lcap) {
+    s.resume = s.phase;
+    s.phase = BPH_OUTPUT;
+    s.flush = 1;
+    s.flushes += 1;
+    s.flush_end = s.level + 1;
+  }
   return s;
 }
 
@@ -394,9 +427,9 @@ __device__ __forceinline__ void push_range(const BArgs &a, unsigned long long *d
 
 template <int BS, int W>
 __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
-  const unsigned long long *fs = level_ptr(a, L);
-  unsigned long long *dst = level_ptr(a, L + 1);
-  const unsigned long long *jf = jf_ptr(a, L);
+  const unsigned long long *fs = level_ptr(a, ring_slot(a, sm.S, 0));
+  unsigned long long *dst = level_ptr(a, ring_slot(a, sm.S, 1));
+  const unsigned long long *jf = jf_ptr(a, ring_slot(a, sm.S, 0));
   const unsigned pw = sm.S.push_words;  // only these words are pushed at this level
   for (int p = blockIdx.x; p < a.n_pieces; p += gridDim.x) {
     const int v = a.piece_v[p];
@@ -467,7 +500,8 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
 // v < n when valid), seen update, counts, and the clearing of lv[L + 2][v]
 // when the next level's push can still reach v.
 template <int W>
-__device__ __forceinline__ void settle_vertex(const BArgs &a, int L, bool valid, long long v, const Mask<W> &sd,
+__device__ __forceinline__ void settle_vertex(const BArgs &a, unsigned long long *clr, bool valid, long long v,
+                                              const Mask<W> &sd,
                                               const Mask<W> &x, long long deg, const unsigned long long *active,
                                               unsigned long long &jw, unsigned long long &mf, unsigned long long &alg,
                                               unsigned long long &munf, Mask<W> &act, WordSums<W> &mfw,
@@ -500,14 +534,15 @@ __device__ __forceinline__ void settle_vertex(const BArgs &a, int L, bool valid,
     Mask<W> z;
 #pragma unroll
     for (int w = 0; w < W; ++w) z.w[w] = 0;
-    store_mask<W>(level_ptr(a, L + 2), v, z);
+    store_mask<W>(clr, v, z);
   }
 }
 
 template <int BS, int W>
 __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
-  unsigned long long *nx = level_ptr(a, L + 1);
-  unsigned long long *jn = jf_ptr(a, L + 1);
+  unsigned long long *nx = level_ptr(a, ring_slot(a, sm.S, 1));
+  unsigned long long *jn = jf_ptr(a, ring_slot(a, sm.S, 1));
+  unsigned long long *clr = level_ptr(a, ring_slot(a, sm.S, 2));  // lv[L + 2]
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
@@ -539,7 +574,7 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
       }
     }
     bool any;
-    settle_vertex<W>(a, L, valid, v, sd, x, deg, sm.S.active, jw, mf, alg, munf, act, mfw, any);
+    settle_vertex<W>(a, clr, valid, v, sd, x, deg, sm.S.active, jw, mf, alg, munf, act, mfw, any);
     const unsigned long long bits = __ballot(any);
     if (lane == 0) jn[wd] = bits;
   }
@@ -553,7 +588,7 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
 // read back only for vertices that gained bits. act is wave-uniform: it is
 // merged with a wave OR only when some lane holds a source not yet in it.
 template <int W>
-__device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, long long v, int ftc,
+__device__ __forceinline__ void settle_bu(const BArgs &a, unsigned long long *clr, bool valid, long long v, int ftc,
                                          const Mask<W> &x, long long deg, unsigned long long &jw,
                                          unsigned long long &mf, unsigned long long &alg, unsigned long long &munf,
                                          Mask<W> &act, WordSums<W> &mfw, bool &any) {
@@ -587,7 +622,7 @@ __device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, lon
       Mask<W> z;
 #pragma unroll
       for (int w = 0; w < W; ++w) z.w[w] = 0;
-      store_mask<W>(level_ptr(a, L + 2), v, z);
+      store_mask<W>(clr, v, z);
     }
   }
 }
@@ -601,10 +636,11 @@ __device__ __forceinline__ void settle_bu(const BArgs &a, int L, bool valid, lon
 // (a top-down launch ran just before), the bits pushed to v are merged in.
 template <int BS, int W, int H>
 __device__ void bbu_phase_h(const BArgs &a, BSmem<BS> &sm, int L) {
-  const unsigned long long *fs = level_ptr(a, L);
-  const unsigned long long *jf = jf_ptr(a, L);  // joint frontier of level L
-  unsigned long long *nx = level_ptr(a, L + 1);
-  unsigned long long *jn = jf_ptr(a, L + 1);
+  const unsigned long long *fs = level_ptr(a, ring_slot(a, sm.S, 0));
+  const unsigned long long *jf = jf_ptr(a, ring_slot(a, sm.S, 0));  // joint frontier of level L
+  unsigned long long *nx = level_ptr(a, ring_slot(a, sm.S, 1));
+  unsigned long long *jn = jf_ptr(a, ring_slot(a, sm.S, 1));
+  unsigned long long *clr = level_ptr(a, ring_slot(a, sm.S, 2));  // lv[L + 2]
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
@@ -717,7 +753,7 @@ __device__ void bbu_phase_h(const BArgs &a, BSmem<BS> &sm, int L) {
       if (valid && pulled) store_mask<W>(nx, v, x);
     }
     bool any;
-    settle_bu<W>(a, L, valid, v, ftc, x, e - s, jw, mf, alg, munf, act, mfw, any);
+    settle_bu<W>(a, clr, valid, v, ftc, x, e - s, jw, mf, alg, munf, act, mfw, any);
     const unsigned long long bits = __ballot(any);
     if (lane == 0) jn[wd] = bits;
   }
@@ -751,18 +787,26 @@ __device__ __forceinline__ unsigned nibble_bytes(unsigned long long m, int k) {
 // Blocks claim segments from a counter of the launch (work): the grid holds
 // more blocks than are resident at once, and blocks that start late take
 // fewer segments instead of an equal share.
+// Levels lo .. hi - 1 are replayed; the table holds L - lo. With lo = 0 the
+// rows are written whole (-1 where unreached); with lo > 0 earlier flushes
+// wrote levels below lo, so only the depths found here are stored. The final
+// pass (not a flush) also counts the reached vertices and edges.
 template <int BS, int W>
-__device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels, unsigned long long *work) {
+__device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, bool final,
+                              unsigned long long *work) {
   constexpr int NWV = BS / kWave;
   const int wid = threadIdx.x / kWave, lane = lane_id();
   unsigned char *d8 = sm.d8[wid];
   unsigned long long edges = 0, verts = 0;
-  // The reached counts leave out the sources a truncated search left
-  // unfinished (their rows are searched again one at a time).
-  const bool trunc = (sm.S.error & BERR_LEVELS) != 0;
-  Mask<W> cm;
-#pragma unroll
-  for (int w = 0; w < W; ++w) cm.w[w] = trunc ? ~sm.S.active[w] : ~0ull;
+  // Ring slot of level L in lo .. hi - 1 (hi - lo < ring): level hi - 1 is
+  // the state's level, so slot(lo) follows without a division, and one wrap
+  // at most.
+  int s0 = sm.S.slot - (hi - 1 - lo);
+  if (s0 < 0) s0 += a.ring;
+  auto slot = [&](int L) {
+    const int t = s0 + (L - lo);
+    return t >= a.ring ? t - a.ring : t;
+  };
   for (;;) {
     if (threadIdx.x == 0) sm.seg = (long long)atomicAdd(work, 1ull);
     __syncthreads();
@@ -771,11 +815,11 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels, unsigne
     if (wd >= a.nwords) break;
     const long long v = wd * 64 + lane;
     const bool valid = v < a.n;
-    if (valid && wid == 0) {
+    if (final && valid && wid == 0) {
       const Mask<W> sd = load_mask<W>(a.seen, v);
       int pc = 0;
 #pragma unroll
-      for (int w = 0; w < W; ++w) pc += __popcll(sd.w[w] & cm.w[w]);
+      for (int w = 0; w < W; ++w) pc += __popcll(sd.w[w]);
       verts += (unsigned long long)pc;
       edges += (unsigned long long)pc * (unsigned long long)(a.off[v + 1] - a.off[v]);
     }
@@ -783,9 +827,9 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels, unsigne
       unsigned int *row = reinterpret_cast<unsigned int *>(d8 + lane * 68);
 #pragma unroll
       for (int j = 0; j < 16; ++j) row[j] = 0xFFFFFFFFu;
-      for (int L0 = 0; L0 < levels; L0 += kWave) {
+      for (int L0 = lo; L0 < hi; L0 += kWave) {
         // Lane j holds the segment's joint-frontier word of level L0 + j.
-        const unsigned long long fl = L0 + lane < levels ? jf_ptr(a, L0 + lane)[wd] : 0ull;
+        const unsigned long long fl = L0 + lane < hi ? a.jf[(long long)slot(L0 + lane) * a.nwords + wd] : 0ull;
         unsigned long long lm = __ballot(fl != 0);
         // Up to kLv levels per round trip: their mask loads are issued
         // together, then replayed (a segment's wavefront visits about six
@@ -807,7 +851,7 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels, unsigne
           for (int t = 0; t < kLv; ++t) {
             if (jj[t] >= 0) {
               const unsigned long long fw = read_lane64(fl, jj[t]);
-              if (((fw >> lane) & 1ull) != 0) mk[t] = level_ptr(a, L0 + jj[t])[v * W + g];
+              if (((fw >> lane) & 1ull) != 0) mk[t] = a.lv[(long long)slot(L0 + jj[t]) * a.nW + v * W + g];
             }
           }
           // Each 4 bits of a mask become 4 table bytes at once (the bits
@@ -818,7 +862,7 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels, unsigne
           if (any) {
             unsigned lrep[kLv];
 #pragma unroll
-            for (int t = 0; t < kLv; ++t) lrep[t] = (unsigned)(L0 + jj[t]) * 0x01010101u;
+            for (int t = 0; t < kLv; ++t) lrep[t] = (unsigned)(L0 - lo + jj[t]) * 0x01010101u;
 #pragma unroll 4
             for (int k = 0; k < 16; ++k) {
               unsigned w = row[k];
@@ -847,18 +891,28 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int levels, unsigne
             typedef int v4i __attribute__((ext_vector_type(4)));
             const unsigned char d0 = d8[(4 * q + 0) * 68 + b], d1 = d8[(4 * q + 1) * 68 + b];
             const unsigned char d2 = d8[(4 * q + 2) * 68 + b], d3 = d8[(4 * q + 3) * 68 + b];
-            v4i x;
-            x.x = d0 == 0xFF ? -1 : int(d0);
-            x.y = d1 == 0xFF ? -1 : int(d1);
-            x.z = d2 == 0xFF ? -1 : int(d2);
-            x.w = d3 == 0xFF ? -1 : int(d3);
-            __builtin_nontemporal_store(x, reinterpret_cast<v4i *>(a.out + (long long)(64 * g + b) * a.n + v0));
+            int *dst = a.out + (long long)(64 * g + b) * a.n + v0;
+            if (lo == 0) {
+              v4i x;
+              x.x = d0 == 0xFF ? -1 : int(d0);
+              x.y = d1 == 0xFF ? -1 : int(d1);
+              x.z = d2 == 0xFF ? -1 : int(d2);
+              x.w = d3 == 0xFF ? -1 : int(d3);
+              __builtin_nontemporal_store(x, reinterpret_cast<v4i *>(dst));
+            } else {
+              if (d0 != 0xFF) dst[0] = lo + int(d0);
+              if (d1 != 0xFF) dst[1] = lo + int(d1);
+              if (d2 != 0xFF) dst[2] = lo + int(d2);
+              if (d3 != 0xFF) dst[3] = lo + int(d3);
+            }
           }
         }
       } else if (valid) {
         for (int b = 0; b < nb; ++b) {
           const unsigned char d = d8[lane * 68 + b];
-          a.out[(long long)(64 * g + b) * a.n + v] = d == 0xFF ? -1 : int(d);
+          int *dst = a.out + (long long)(64 * g + b) * a.n + v;
+          if (lo == 0) *dst = d == 0xFF ? -1 : int(d);
+          else if (d != 0xFF) *dst = lo + int(d);
         }
       }
       wave_sync();
@@ -884,7 +938,9 @@ __global__ __launch_bounds__(BS) void k_blevel(BArgs a, long long i) {
   switch (S.phase) {
     case BPH_TD: btd_phase<BS, W>(a, sm, S.level); break;
     case BPH_APPLY: bapply_phase<BS, W>(a, sm, S.level); break;
-    default: boutput_phase<BS, W>(a, sm, S.levels, &a.ctl->cnt[slot].v[C_WORK]); break;
+    default:
+      boutput_phase<BS, W>(a, sm, S.base, S.flush ? S.flush_end : S.levels, !S.flush, &a.ctl->cnt[slot].v[C_WORK]);
+      break;
   }
   if (S.phase != BPH_TD) block_counts_flush(a, sm, slot);
 }
@@ -923,7 +979,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(bpersist_wav
       case BPH_TD: btd_phase<BS, W>(a, sm, S.level); break;
       case BPH_APPLY: bapply_phase<BS, W>(a, sm, S.level); break;
       case BPH_BU: bbu_phase<BS, W>(a, sm, S.level); break;
-      default: boutput_phase<BS, W>(a, sm, S.levels, &a.ctl->cnt[slot].v[C_WORK]); break;
+      default:
+        boutput_phase<BS, W>(a, sm, S.base, S.flush ? S.flush_end : S.levels, !S.flush,
+                             &a.ctl->cnt[slot].v[C_WORK]);
+        break;
     }
     if (S.phase != BPH_TD) block_counts_flush(a, sm, slot);
     const bool ok = grid_sync(&a.ctl->sync, 400000000ull);
@@ -965,6 +1024,7 @@ __global__ void k_binit_zero(BArgs a, const int *roots) {
   if (gt == 0) {
     BState S{};
     S.level = -1;
+    S.slot = a.ring - 1;
     S.phase = BPH_APPLY;
     a.ctl->st[0] = S;
     a.ctl->st[1] = S;
@@ -1142,6 +1202,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   a.nwords = int(g->nwords);
   a.k = k;
   a.lcap = b->lcap;
+  a.ring = b->lcap + 3;
   a.n_hub_block = g->n_hub_block;
   a.n_pieces = g->n_pieces;
   const ent_bfs_config &c = g->cfg;
@@ -1211,7 +1272,9 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   const int lgrid = c.grid_blocks > 0 ? c.grid_blocks : g->cus * std::max(1, c.blocks_per_cu > 0 ? c.blocks_per_cu : 8);
   long long i = 0;
   int kk = std::max(2, b->expect_launches);
-  const long long max_launches = 4LL * (b->lcap + 4);
+  // At most two launches per level (top-down, then pull or apply), one flush
+  // per lcap + 1 levels and the output; levels <= n.
+  const long long max_launches = 4LL * ((long long)g->n + 4);
   for (;;) {
     for (int j = 0; j < kk; ++j, ++i) {
       void *args[] = {&a, &i};
@@ -1271,25 +1334,11 @@ int batch_locked(ent_graph *g, const int32_t *roots, int k, int32_t *depths_dev,
     ignore(hipStreamSynchronize(g->stream));
     return set_error(ENT_EDEVICE, "persistent multi-source search: grid barrier wait timed out");
   }
-  if (fin.error & ~BERR_LEVELS) {
+  if (fin.error) {
     ignore(hipStreamSynchronize(g->stream));
     return set_error(ENT_EINTERNAL, "device reported error flags 0x%x", fin.error);
   }
-  unsigned long long edges = fin.total_edges, verts = fin.total_vertices, alg = fin.alg_bytes;
-  int fallback = 0;
-  if (fin.error & BERR_LEVELS) {
-    for (int s = 0; s < k; ++s) {
-      if (!((fin.active[s >> 6] >> (s & 63)) & 1ull)) continue;
-      ent_stats one;
-      rc = search_locked(g, roots[s], depths_dev + size_t(s) * size_t(g->n), &one, t0);
-      if (rc) return rc;
-      edges += (unsigned long long)one.reached_edges;
-      verts += (unsigned long long)one.reached_vertices;
-      alg += (unsigned long long)one.alg_bytes;
-      fin.levels = std::max(fin.levels, one.levels);
-      ++fallback;
-    }
-  }
+  const unsigned long long edges = fin.total_edges, verts = fin.total_vertices, alg = fin.alg_bytes;
   HIP_TRY(hipEventRecord(g->ev1, g->stream));
   HIP_TRY(hipEventSynchronize(g->ev1));
   const auto t1 = std::chrono::steady_clock::now();
@@ -1307,7 +1356,7 @@ int batch_locked(ent_graph *g, const int32_t *roots, int k, int32_t *depths_dev,
     st->host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     st->bu_level_mask = fin.bu_mask;
     st->td_level_mask = fin.td_mask;
-    st->fallback_rows = fallback;
+    st->flushes = fin.flushes;
   }
   return ENT_OK;
 }

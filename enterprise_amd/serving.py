@@ -15,9 +15,9 @@ max_batch queued queries (FIFO) and runs them as one multi-source search
 single-source searches one after another (ent_bfs_device): a multi-source
 level sweeps every vertex, which costs more than a few direction-optimizing
 single-source searches when the batch is that small. A batch deeper than the
-slot's reserved levels still completes: the library searches the sources it
-left unfinished again on the single-source path (ent_batch_stats.fallback_rows),
-so one deep query does not fail the queries batched with it. Measured on R-MAT 26 +
+slot's reserved levels still runs as one batch: the library writes the kept
+levels to the rows and reuses their slots (ent_batch_stats.flushes), so one
+deep query does not fail or split the batch it is in. Measured on R-MAT 26 +
 R-MAT 20 with four slots per graph, single-source searches alone saturate at
 about 520 queries/s and batches at about 1,150; plain_max = 12 gave the lowest
 latencies below saturation. Queries that arrive while other batches run are
@@ -97,7 +97,7 @@ class _GraphLane:
         self.queue = collections.deque()
         self.cv = threading.Condition()
         self.stats = {"batches": 0, "plain_batches": 0, "queries": 0, "reached_edges": 0, "alg_bytes": 0,
-                      "busy_s": 0.0, "batch_sizes": [], "fallback_rows": 0}
+                      "busy_s": 0.0, "batch_sizes": [], "flushes": 0}
 
 
 class BFSServer:
@@ -262,7 +262,7 @@ class BFSServer:
                     s["alg_bytes"] += st["alg_bytes"]
                     s["busy_s"] += t1 - t0
                     s["batch_sizes"].append(len(batch))
-                    s["fallback_rows"] += int(st.get("fallback_rows", 0))
+                    s["flushes"] += int(st.get("flushes", 0))
             except Exception as e:  # the batch fails, the slot stays usable
                 t1 = time.perf_counter()
                 for q in batch:

@@ -10,13 +10,12 @@
  *
  * Memory: besides the graph, a workspace of (max_levels + 3) * (n * W * 8 +
  * ceil(n / 64) * 8) bytes holds the per-level frontier masks and joint
- * frontier bitmaps (levels are replayed at the end to write every output
- * row in full, coalesced lines). A search that needs
- * more joint levels than reserved still returns every row complete: the
- * rows of sources that finished within the reserved levels come from the
- * multi-source search, and each source still unfinished is searched again on
- * its own (the single-source path, ent_bfs_device) into its row; their
- * number is ent_batch_stats.fallback_rows.
+ * frontier bitmaps of the levels not yet written to the output (levels are
+ * replayed in a pass that writes the output rows in full, coalesced lines).
+ * The search itself has no depth limit: when it would need more levels than
+ * are kept, a flush pass writes the kept levels to the rows and their slots
+ * are reused (ent_batch_stats.flushes), so a high-diameter graph runs as one
+ * batch in the same workspace.
  */
 #ifndef ENT_BATCH_H
 #define ENT_BATCH_H
@@ -48,13 +47,15 @@ typedef struct ent_batch_stats {
   uint64_t td_level_mask;   /* bit l: level l pushed (top-down) some of its
                                words; both bits set: the level split its words
                                by frontier size */
-  int32_t fallback_rows;    /* rows searched again on the single-source path
-                               because the search ran out of reserved levels */
+  int32_t flushes;          /* passes that wrote the kept levels to the rows
+                               before the search ended (one per max_levels + 1
+                               levels beyond the first max_levels) */
   int32_t pad;
 } ent_batch_stats;
 
-/* Reserves the workspace for up to k sources and max_levels joint levels
- * (0 = keep the current reservation or the default of 32 levels). */
+/* Reserves the workspace for up to k sources, keeping max_levels joint
+ * levels between flushes (1..250; 0 = keep the current reservation or the
+ * default of 32 levels). */
 int ent_batch_reserve(ent_graph *g, int32_t k, int32_t max_levels);
 
 /* Frees the multi-source workspace (the graph stays resident). */

@@ -86,26 +86,40 @@ def test_batch_closed_forms(oracle, name):
             assert np.array_equal(out[i], np.asarray(depth_fn(int(r)), dtype=np.int32)), (name, r)
 
 
+def expected_flushes(levels, kept):
+    """Flush passes of a search of `levels` joint levels keeping `kept`
+    levels: level l starts with a flush of base .. l when l - base > kept."""
+    base = flushes = 0
+    for lv in range(1, levels):
+        if lv - base > kept:
+            flushes += 1
+            base = lv + 1
+    return flushes
+
+
 @pytest.mark.parametrize("launch", ["stream", "persistent"])
-def test_batch_beyond_level_capacity(oracle, launch):
-    """A search deeper than the reserved levels still returns every row: the
-    sources that finished within the reserved levels come from the batch,
-    the unfinished ones are searched again on the single-source path."""
+def test_batch_deeper_than_kept_levels(oracle, launch):
+    """A search deeper than the kept levels runs as one batch: flush passes
+    write the kept levels to the rows (whole rows the first time, then only
+    the depths each pass found) and their slots are reused."""
     form = {"stream": L.LAUNCH_STREAM, "persistent": L.LAUNCH_PERSISTENT}[launch]
     p300, pdepth = graphs.path(300)
     with DeviceGraph(p300) as dp:
         dp.configure(launch=form)
         dp.batch_reserve(4, max_levels=10)
-        _say(f"[gpu] batch beyond capacity: path300 {launch}")
+        _say(f"[gpu] batch deeper than kept levels: path300 {launch}")
         out, st = dp.bfs_batch([0, 1, 2, 299])
         for i, r in enumerate([0, 1, 2, 299]):
             assert np.array_equal(out[i], np.asarray(pdepth(r), dtype=np.int32)), r
-        assert st["fallback_rows"] == 4 and st["levels"] == 300
+        assert st["levels"] == 300 and st["flushes"] == expected_flushes(300, 10) == 25  # at levels 11, 23, .., 299
+        out, st = dp.bfs_batch([150])            # a second search on the same ring
+        assert np.array_equal(out[0], np.asarray(pdepth(150), dtype=np.int32))
+        assert st["flushes"] == expected_flushes(151, 10)
         dp.batch_reserve(4, max_levels=250)
-        out, st = dp.bfs_batch([100, 150])      # 200 levels fit
-        assert out[0][299] == 199 and out[1][0] == 150 and st["fallback_rows"] == 0
+        out, st = dp.bfs_batch([100, 150])      # 200 levels kept
+        assert out[0][299] == 199 and out[1][0] == 150 and st["flushes"] == 0
     # One deep source among 63 shallow ones: a 400-vertex path next to a
-    # 2,000-leaf star; only the path end's row is searched again.
+    # 2,000-leaf star.
     n1, n2 = 400, 2000
     e = [(i, i + 1) for i in range(n1 - 1)] + [(n1, n1 + j) for j in range(1, n2)]
     g = graphs.edges_graph(n1 + n2, e)
@@ -115,7 +129,7 @@ def test_batch_beyond_level_capacity(oracle, launch):
         dg.batch_reserve(64, max_levels=40)
         _say(f"[gpu] batch beyond capacity: path+star {launch}")
         out, st = dg.bfs_batch(roots)
-        assert st["fallback_rows"] == 1, st
+        assert st["flushes"] == expected_flushes(n1, 40), st
         for i, r in enumerate(roots):
             assert np.array_equal(out[i], oracle.bfs(g.beg_pos, g.adj, int(r))), (i, r)
         deg = g.degrees()
@@ -269,3 +283,26 @@ def test_batch_directed_topdown(oracle):
             assert np.any(out[:, 3000:] > 0)                 # sinks reached
             assert st["levels"] == int(out.max()) + 1
             assert n == 4000
+
+
+@pytest.mark.parametrize("launch", ["stream", "persistent"])
+def test_batch_grid_many_flushes(oracle, launch):
+    """A 64-source batch on a 200 x 40 grid (about 240 levels) keeping 16
+    levels: more than a dozen flush passes, every row against the oracle and
+    the reached counts."""
+    form = {"stream": L.LAUNCH_STREAM, "persistent": L.LAUNCH_PERSISTENT}[launch]
+    g = HostGraph.grid(200, 40, delete_ppm=20000, seed=240, keep_lcc=True)
+    roots = g.pick_roots(64, 241)
+    with DeviceGraph(g) as dg:
+        dg.configure(launch=form)
+        dg.batch_reserve(64, max_levels=16)
+        _say(f"[gpu] batch grid200x40 {launch}")
+        out, st = dg.bfs_batch(roots)
+        for i, r in enumerate(roots):
+            assert np.array_equal(out[i], oracle.bfs(g.beg_pos, g.adj, int(r))), (i, int(r))
+        assert st["levels"] == int(out.max()) + 1 and st["levels"] > 150
+        assert st["flushes"] == expected_flushes(st["levels"], 16) > 10
+        deg = g.degrees()
+        reached = out >= 0
+        assert st["reached_vertices"] == int(reached.sum())
+        assert st["reached_edges"] == int((reached * deg[None, :]).sum())

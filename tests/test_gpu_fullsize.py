@@ -3,7 +3,8 @@ R-MAT scale 22 (64-root workload, tuned single-source configuration) against
 the oracle bit for bit; the 512-root multi-source search on R-MAT scale 23
 (tuned multi-source knobs) against the single-source path on every row and
 the oracle on sampled rows; the 4096 x 4096 grid in every launch form against
-the oracle; a complete bipartite graph with more than 2^31 directed edges
+the oracle, and as one 64-source batch (over a hundred flushes of the level
+ring); a complete bipartite graph with more than 2^31 directed edges
 (int64 offsets end to end) in every launch form and a 64-source batch; and
 the serving workload (R-MAT 26 + R-MAT 20, Poisson arrivals) with every query
 checked against the single-source path and sampled queries against the
@@ -80,7 +81,7 @@ def test_rmat23_batch_512_rows(oracle):
             h = dg.row_hash(out, k, g.n)
             assert np.array_equal(h, plain)
             assert st["bu_level_mask"] & st["td_level_mask"]   # some level split its words
-            assert st["fallback_rows"] == 0
+            assert st["flushes"] == 0
             for i in (0, 255, 256, 511):
                 want = oracle.bfs(g.beg_pos, g.adj, int(roots[i]))
                 assert int(h[i]) == int(row_hash_host(want)), i
@@ -111,6 +112,38 @@ def test_grid4096_forms(oracle):
         for r2 in roots:
             dg.bfs_device(int(r2))
             assert dg.validate_device(int(r2)) == [0, 0, 0, 0], int(r2)
+
+
+def test_grid4096_batch_64(oracle):
+    """The 4096 x 4096 grid (about 5,900 levels) as one 64-source batch
+    keeping 48 levels: over a hundred flush passes. Every row's checksum
+    equals the single-source result for its root, two rows equal the oracle
+    bit for bit, and the reached counts match the rows."""
+    g = HostGraph.grid(4096, 4096, delete_ppm=20000, seed=4096, keep_lcc=True)
+    roots = g.pick_roots(64, 1007)
+    with DeviceGraph(g) as dg:
+        out = dg.device_alloc(64 * g.n * 4)
+        try:
+            dg.batch_reserve(64, 48)
+            _say("[gpu] grid4096 64-source batch")
+            t0 = time.perf_counter()
+            st = dg.bfs_batch_device(roots, out)
+            _say(f"[gpu] grid4096 batch: {st['levels']} levels, {st['flushes']} flushes, "
+                 f"{time.perf_counter() - t0:.2f}s")
+            assert st["levels"] > 5000 and st["flushes"] > 100
+            h = dg.row_hash(out, 64, g.n)
+            edges = verts = 0
+            for i, r in enumerate(roots):
+                s1 = dg.bfs_device(int(r))
+                assert int(dg.row_hash(dg.depth_device_ptr(), 1, g.n)[0]) == int(h[i]), i
+                edges += s1["reached_edges"]
+                verts += s1["reached_vertices"]
+            for i in (0, 63):
+                want = oracle.bfs(g.beg_pos, g.adj, int(roots[i]))
+                assert int(h[i]) == int(row_hash_host(want)), i
+            assert (st["reached_edges"], st["reached_vertices"]) == (edges, verts)
+        finally:
+            dg.device_free(out)
 
 
 def _bipartite(a):
@@ -172,7 +205,7 @@ def test_offsets_past_int32():
         out, st = dg.bfs_batch(roots)
         for i, r in enumerate(roots):
             assert np.array_equal(out[i], want(int(r))), (i, int(r))
-        assert st["reached_edges"] == 64 * m and st["fallback_rows"] == 0
+        assert st["reached_edges"] == 64 * m and st["flushes"] == 0
 
 
 def test_serving_rmat26_rmat20(oracle):

@@ -125,7 +125,8 @@ def test_memory_budget_admission(two_graphs):
 def test_deep_query_in_batch(oracle):
     """One query deeper than the slot's reserved levels (the end of a
     400-vertex path) served in one 64-query multi-source batch with 63
-    shallow ones (leaves of a star): every query gets its correct depths."""
+    shallow ones (leaves of a star): the batch flushes its kept levels to the
+    rows as it goes, and every query gets its correct depths."""
     n1, n2 = 400, 3000
     e = [(i, i + 1) for i in range(n1 - 1)] + [(n1, n1 + j) for j in range(1, n2)]
     g = graphs.edges_graph(n1 + n2, e)
@@ -141,7 +142,7 @@ def test_deep_query_in_batch(oracle):
             for q in qs:
                 q.wait(120)
             st = srv.lanes["g"].stats
-            assert st["batch_sizes"] == [64] and st["fallback_rows"] == 1, st
+            assert st["batch_sizes"] == [64] and st["flushes"] > 0, st
             for q in qs:
                 want = oracle.bfs(g.beg_pos, g.adj, q.root)
                 assert np.array_equal(q.depth, want), q.root
