@@ -15,8 +15,8 @@ levels (bfs_gpu_opt.cuh:413-445).
 The default line also carries, under "extra", the 512-source multi-source
 search on R-MAT scale 23 against the plain path on the same roots
 (extra.batch) and the per-level cost of each launch form on the 4096^2
-high-diameter grid (extra.grid), each with its oracle parity and CPU figures
-(--no-extra skips both).
+high-diameter grid, with the same grid as one 64-source batch (extra.grid),
+each with its oracle parity and CPU figures (--no-extra skips both).
 
 Other modes (--mode): grid, batch, serve (continuous-batching serving loop)
 and csr (CSR construction on the device from the raw tuples of R-MAT scale
@@ -338,7 +338,7 @@ def bench_plain(args):
         log("[bench] extra: 4096^2 grid, launch forms")
         gr = run_extra(args, "grid", grid_roots=0.25, set=[])
         out["extra"]["grid"] = {k: gr[k] for k in (
-            "value", "unit", "best_form", "forms", "device_validation", "cpu_serial_ms_per_bfs",
+            "value", "unit", "best_form", "forms", "device_validation", "batch", "cpu_serial_ms_per_bfs",
             "parity_roots", "parity_forms", "parity_mismatches", "config") if k in gr}
     return out
 
@@ -349,6 +349,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10,
                     help="plain mode: sweeps over the 64 roots (one step = one sweep)")
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-grid-batch", action="store_true",
+                    help="grid mode: skip the 64-source batch of the grid")
     ap.add_argument("--no-extra", action="store_true",
                     help="plain mode: skip the batch and grid figures after the plain line")
     ap.add_argument("--grid-roots", type=float, default=1.0,
@@ -362,7 +364,15 @@ def main():
     ap.add_argument("--no-examined", action="store_true",
                     help="skip the examined-work pass (counting kernel variant) after the timed searches")
     ap.add_argument("--serve-scale", type=int, default=26)
-    ap.add_argument("--serve-budget-gb", type=int, default=160)
+    ap.add_argument("--serve-budget-gb", type=int, default=256,
+                    help="serve mode: device memory for the slots, GiB (the MI355X has 288 GB)")
+    ap.add_argument("--serve-batch", type=int, default=512,
+                    help="serve mode: queries per batch on each graph's first slot")
+    ap.add_argument("--serve-small-batch", type=int, default=64,
+                    help="serve mode: queries per batch on the other slots (0: all slots --serve-batch)")
+    ap.add_argument("--serve-levels", type=int, default=8,
+                    help="serve mode: levels a slot keeps between flushes to the rows")
+    ap.add_argument("--serve-slots", type=int, default=3, help="serve mode: most slots per graph")
     ap.add_argument("--rates", default="100,200,500,1000,2000,5000")
     ap.add_argument("--duration", type=float, default=2.0)
     ap.add_argument("--plain-max", type=int, default=12,

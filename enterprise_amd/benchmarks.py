@@ -189,13 +189,40 @@ def grid(args):
     for r in roots:
         dg.bfs_device(int(r))
         violations += sum(dg.validate_device(int(r)))
-    dg.free()
     best = min(res, key=lambda k: res[k]["ms_per_bfs"])
+    # The same graph as one 64-source batch keeping 48 levels (the level ring
+    # flushes to the rows about every 49 levels), against 64 searches at the
+    # best form's mean time; four rows checked against the single-source path.
+    batch = None
+    if not getattr(args, "no_grid_batch", False):
+        k = 64
+        broots = g.pick_roots(k, 1007)
+        out = dg.device_alloc(k * g.n * 4)
+        try:
+            dg.batch_reserve(k, 48)
+            t1 = time.perf_counter()
+            st = dg.bfs_batch_device(broots, out)
+            bms = (time.perf_counter() - t1) * 1e3
+            h = dg.row_hash(out, k, g.n)
+            mism = 0
+            for i in (0, 21, 42, 63):
+                dg.bfs_device(int(broots[i]))
+                mism += int(int(dg.row_hash(dg.depth_device_ptr(), 1, g.n)[0]) != int(h[i]))
+        finally:
+            dg.device_free(out)
+            dg.batch_release()
+        batch = {"k": k, "roots_seed": 1007, "batch_ms": round(bms, 1), "levels": st["levels"],
+                 "flushes": st["flushes"], "kept_levels": 48,
+                 "speedup_vs_plain": round(k * res[best]["ms_per_bfs"] / bms, 2),
+                 "rows_checked": 4, "rows_mismatching_plain": mism}
+        log(f"[grid] batch of {k}: {batch}")
+    dg.free()
     return {"mode": "grid", "metric": "us per BFS level, high-diameter grid", "unit": "us/level",
             "value": res[best]["us_per_level"], "best_form": best, "higher_is_better": False,
             "config": {"workload": f"{side}x{side} 4-neighbour grid, 2% edges deleted (seed 4096), "
                                    "largest component, 16 roots (seed 1003)", "n": g.n, "m": g.m},
             "forms": res, "device_validation": {"roots": len(roots), "violations": int(violations)},
+            "batch": batch,
             "_cpu_check": {"graph": g, "checks": checks}}
 
 
@@ -205,10 +232,13 @@ MODES["grid"] = grid
 def serve(args):
     """Config 5: a continuous-batching server over two resident graphs, R-MAT
     scale 26 (80% of queries) and scale 20, queries arriving as a seeded
-    Poisson stream at each rate, uniform non-isolated roots, up to 64 queries
-    per multi-source batch. Reports p50/p99 latency (submit to end of the
-    serving batch) and GTEPS per rate; sampled queries are checked against
-    the single-source path by device row checksums."""
+    Poisson stream at each rate, uniform non-isolated roots. Each graph's
+    first slot takes batches of up to --serve-batch queries (default 512),
+    the others up to --serve-small-batch (default 64) while the queue fits
+    in them; every slot keeps --serve-levels levels between flushes
+    (default 8; R-MAT 26 searches rarely need more). Reports p50/p99 latency
+    (submit to end of the serving batch) and GTEPS per rate; sampled queries
+    are checked against the single-source path by device row checksums."""
     import sys
 
     import numpy as np
@@ -236,8 +266,11 @@ def serve(args):
             d.configure(**{k: v for k, v in tuned.items() if k not in ("reserved", "launch", "hub_cache")})
     m_big = gb.m
     del gb  # the host copy is no longer needed
-    srv = BFSServer(mem_budget_bytes=int(getattr(args, "serve_budget_gb", 160)) << 30, max_batch=64,
-                    max_levels=28, max_slots=4, plain_max=int(getattr(args, "plain_max", 12)))
+    small = int(getattr(args, "serve_small_batch", 64) or 0)
+    srv = BFSServer(mem_budget_bytes=int(getattr(args, "serve_budget_gb", 256)) << 30,
+                    max_batch=int(getattr(args, "serve_batch", 512)), max_levels=int(getattr(args, "serve_levels", 8)),
+                    max_slots=int(getattr(args, "serve_slots", 3)), plain_max=int(getattr(args, "plain_max", 12)),
+                    small_batch=small or None)
     slots_big = srv.add_graph("large", db, share_budget=0.9)
     slots_small = srv.add_graph("small", ds, share_budget=0.1)
     log(f"[serve] slots: large {slots_big}, small {slots_small}")
@@ -280,7 +313,8 @@ def serve(args):
         ds.free()
     return {"mode": "serve", "metric": "p50/p99 query latency vs GTEPS", "unit": "ms",
             "config": {"workload": f"serving R-MAT {big_scale} (80%) + R-MAT 20, Poisson arrivals, "
-                                   "64-source continuous batching", "m_large": int(m_big),
+                                   f"continuous batching of up to {srv.max_batch} queries", "m_large": int(m_big),
+                       "max_batch": srv.max_batch, "small_batch": srv.small_batch, "kept_levels": srv.max_levels,
                        "slots": {"large": slots_big, "small": slots_small}, "duration_s": dur,
                        "plain_max": srv.plain_max},
             "rates": per_rate}

@@ -17,10 +17,13 @@ level sweeps every vertex, which costs more than a few direction-optimizing
 single-source searches when the batch is that small. A batch deeper than the
 slot's reserved levels still runs as one batch: the library writes the kept
 levels to the rows and reuses their slots (ent_batch_stats.flushes), so one
-deep query does not fail or split the batch it is in. Measured on R-MAT 26 +
-R-MAT 20 with four slots per graph, single-source searches alone saturate at
-about 520 queries/s and batches at about 1,150; plain_max = 12 gave the lowest
-latencies below saturation. Queries that arrive while other batches run are
+deep query does not fail or split the batch it is in. With small_batch, a
+graph's first slot takes batches of up to max_batch and the others up to
+small_batch, and a smaller slot takes work only while the whole queue fits
+in it. Measured on R-MAT 26 + R-MAT 20: single-source searches alone
+saturate at about 520 queries/s, four 64-query slots at about 1,150, and one
+512-query slot with two 64-query slots (8 kept levels) at about 3,300;
+plain_max = 12 gave the lowest latencies below saturation. Queries that arrive while other batches run are
 admitted in the next free slot. Batches on different slots run concurrently
 on the device. Each query's result is its depth row in the slot's device
 buffer. The row's device checksum (ent_device_row_hash) is returned, and
@@ -78,6 +81,7 @@ class _Slot:
     def __init__(self, owner: DeviceGraph, max_batch, max_levels, share=True):
         self.dg = DeviceGraph(share=owner) if share else owner
         self.shared = share
+        self.capacity = max_batch  # most queries per batch on this slot
         self.dg.batch_reserve(max_batch, max_levels)
         self.out = self.dg.device_alloc(max_batch * owner.n * 4)
 
@@ -104,11 +108,17 @@ class BFSServer:
     """Continuous-batching server for BFS queries over resident graphs."""
 
     def __init__(self, mem_budget_bytes=None, max_batch=64, max_levels=32, max_queue=100000,
-                 max_slots=4, plain_max=12):
+                 max_slots=4, plain_max=12, small_batch=None):
         if not 1 <= max_batch <= 512:
             raise ValueError("max_batch must be in [1, 512]")
+        if small_batch is not None and not 1 <= small_batch <= max_batch:
+            raise ValueError("small_batch must be in [1, max_batch]")
         self.mem_budget = mem_budget_bytes
         self.max_batch = max_batch
+        # With small_batch, each graph's first slot takes batches of up to
+        # max_batch (throughput under load) and the others up to small_batch,
+        # so that more batches run side by side for the same memory.
+        self.small_batch = small_batch
         self.max_levels = max_levels
         self.max_queue = max_queue
         self.max_slots = max_slots
@@ -121,31 +131,36 @@ class BFSServer:
         self._lock = threading.Lock()
 
     # -- set-up ---------------------------------------------------------------
-    def slot_bytes(self, dg: DeviceGraph):
-        ws = L.lib().ent_batch_workspace_bytes(dg.handle, self.max_batch, self.max_levels)
+    def slot_bytes(self, dg: DeviceGraph, batch=None):
+        batch = self.max_batch if batch is None else batch
+        ws = L.lib().ent_batch_workspace_bytes(dg.handle, batch, self.max_levels)
         info = dg.info()
         # Output rows + multi-source workspace + the shared handle's own
         # single-source workspace (depth, bitmaps, marks, queues).
         per_handle = info["n"] * 4 + info["n"] + 3 * ((info["n"] + 63) // 64) * 8 + 2 * 64 * 4096 * 4
-        return self.max_batch * info["n"] * 4 + int(ws) + per_handle
+        return batch * info["n"] * 4 + int(ws) + per_handle
 
     def add_graph(self, name, dg: DeviceGraph, share_budget=1.0, copy_rows=False, slots=None):
         """Registers a resident graph. The number of slots follows from the
         memory budget (share_budget of it for this graph), capped by
-        max_slots; at least one slot is required."""
+        max_slots; at least one slot is required. With small_batch, the first
+        slot holds max_batch rows and the others small_batch."""
         per = self.slot_bytes(dg)
+        per_small = self.slot_bytes(dg, self.small_batch) if self.small_batch else per
         if slots is None:
             if self.mem_budget is None:
                 slots = self.max_slots
             else:
-                slots = int(min(self.max_slots, (self.mem_budget * share_budget) // per))
+                avail = self.mem_budget * share_budget
+                slots = 0 if avail < per else int(1 + min(self.max_slots - 1, (avail - per) // per_small))
         if slots < 1:
             raise L.EntError(L.ENT_ENOMEM, f"graph {name}: one slot needs {per} bytes, budget "
                                            f"{self.mem_budget * share_budget:.0f}")
+        caps = [self.max_batch] + [self.small_batch or self.max_batch] * (slots - 1)
         made = []
         try:
-            for i in range(slots):
-                made.append(_Slot(dg, self.max_batch, self.max_levels, share=True))
+            for cap in caps:
+                made.append(_Slot(dg, cap, self.max_levels, share=True))
         except Exception:
             for s in made:
                 s.close()
@@ -218,11 +233,15 @@ class BFSServer:
         n = lane.dg.n
         while True:
             with lane.cv:
-                while not lane.queue and not self._stop:
+                # A smaller slot takes work only while the whole queue fits in
+                # it: under load the queue goes to the largest slot in big
+                # batches (several batches side by side slow each other down).
+                while not self._stop and (not lane.queue or (slot.capacity < lane.max_batch
+                                                              and len(lane.queue) > slot.capacity)):
                     lane.cv.wait(0.05)
                 if self._stop and not lane.queue:
                     return
-                batch = [lane.queue.popleft() for _ in range(min(lane.max_batch, len(lane.queue)))]
+                batch = [lane.queue.popleft() for _ in range(min(slot.capacity, len(lane.queue)))]
             with self._lock:
                 self._batch_id += 1
                 bid = self._batch_id

@@ -122,6 +122,44 @@ def test_memory_budget_admission(two_graphs):
     srv.close()
 
 
+def test_mixed_slot_capacities(oracle, two_graphs):
+    """small_batch: the first slot takes batches of up to max_batch, the
+    others up to small_batch; the budget is charged for each size. The
+    smaller slots take work only while the whole queue fits in them, so a
+    burst queued before start goes to the large slot in batches beyond
+    small_batch; a trickle afterwards is served too. Every row equals the
+    single-source result (sampled rows equal the oracle)."""
+    hg, dg = two_graphs["big"]
+    probe = BFSServer(max_batch=128, max_levels=12, small_batch=16)
+    big, small = probe.slot_bytes(dg), probe.slot_bytes(dg, 16)
+    assert small < big
+    srv = BFSServer(mem_budget_bytes=big + 2 * small + small // 2, max_batch=128, max_levels=12,
+                    max_slots=4, plain_max=4, small_batch=16)
+    assert srv.add_graph("g", dg, copy_rows=True) == 3
+    assert [s.capacity for s in srv.lanes["g"].slots] == [128, 16, 16]
+    try:
+        roots = hg.pick_roots(300, 17)
+        qs = [srv.submit("g", int(r)) for r in roots]   # queued before start
+        _say("[gpu] serving: mixed slot capacities")
+        srv.start()
+        for q in qs:
+            q.wait(120)
+        sizes = srv.lanes["g"].stats["batch_sizes"]
+        assert sum(sizes) == len(qs) and max(sizes) > 16 and max(sizes) <= 128, sizes
+        for r in roots[:20]:                      # a trickle: one query at a time
+            q = srv.submit("g", int(r))
+            q.wait(60)
+            qs.append(q)
+        for i, q in enumerate(qs):
+            assert q.error is None
+            want, _ = dg.bfs(q.root)
+            assert np.array_equal(q.depth, want), q.root
+            if i % 60 == 0:
+                assert np.array_equal(q.depth, oracle.bfs(hg.beg_pos, hg.adj, q.root)), q.root
+    finally:
+        srv.close()
+
+
 def test_deep_query_in_batch(oracle):
     """One query deeper than the slot's reserved levels (the end of a
     400-vertex path) served in one 64-query multi-source batch with 63
