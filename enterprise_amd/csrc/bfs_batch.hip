@@ -119,6 +119,7 @@ struct BArgs {
   unsigned long long *seen;
   unsigned long long *lv;
   unsigned long long *jf;  // ring of lcap + 3 joint-frontier bitmaps of nwords
+  const unsigned long long *nz;  // bit v: degree(v) > 0
   int *out;
   BControl *ctl;
   long long m;
@@ -652,7 +653,9 @@ __device__ void bbu_phase_h(const BArgs &a, BSmem<BS> &sm, int L) {
   const bool merge = sm.S.push_words != 0;
   for (long long wd = (long long)blockIdx.x * (BS / kWave) + threadIdx.x / kWave; wd < a.nwords; wd += nwv) {
     const long long v = wd * 64 + lane;
-    const bool valid = v < a.n;
+    // A vertex without edges can neither pull nor be pushed to (the pull
+    // runs only on a symmetric adjacency): nothing of it is loaded.
+    const bool valid = v < a.n && ((a.nz[wd] >> lane) & 1ull) != 0;
     Mask<W> tgt, acc;
     bool need = false;
     int ftc = 0;
@@ -1189,6 +1192,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   a.off = g->off;
   a.adj = g->adj;
   a.hub_block = g->hub_block;
+  a.nz = g->nzmask;
   a.piece_v = g->piece_v;
   a.piece_k = g->piece_k;
   a.seen = b->seen;
