@@ -1100,7 +1100,7 @@ __global__ void k_zero_u64(unsigned long long *p, long long n) {
 
 struct BatchWorkspace {
   int words = 0;      // W (1, 2, 4 or 8)
-  int lcap = 0;       // max joint levels
+  int lcap = 0;       // levels kept between flushes (ring of lcap + 3 slots)
   int64_t n = 0;
   unsigned long long *seen = nullptr;
   unsigned long long *lv = nullptr;
@@ -1318,10 +1318,9 @@ void batch_release(ent_graph *g) {
 
 namespace {
 
-// Multi-source search with g->mu held. A search that runs out of reserved
-// levels writes the rows its levels settled, and the sources it left
-// unfinished are searched again one at a time into their rows (counted in
-// st->fallback_rows), so every row is complete whatever the depth.
+// Multi-source search with g->mu held. Any depth fits the reserved
+// workspace: the level ring flushes kept levels to the rows as it goes
+// (st->flushes).
 int batch_locked(ent_graph *g, const int32_t *roots, int k, int32_t *depths_dev, ent_batch_stats *st,
                  std::chrono::steady_clock::time_point t0) {
   HIP_TRY(hipSetDevice(g->device));

@@ -1793,7 +1793,8 @@ int copy_to_host(ent_graph *g, void *dst, const void *src, size_t bytes) {
 // One single-source search from root into depth (device memory), with the
 // handle's lock held and ev0 already recorded; fills *st (may be NULL). t0 is
 // the start of the caller's host window.
-int search_locked(ent_graph *g, int root, int *depth, ent_stats *st, std::chrono::steady_clock::time_point t0) {
+static int search_locked(ent_graph *g, int root, int *depth, ent_stats *st,
+                         std::chrono::steady_clock::time_point t0) {
   State c{};
   g->trace_launches = 0;
   int rc = run_search(g, root, depth, &c);

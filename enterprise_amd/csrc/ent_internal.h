@@ -5,7 +5,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cstdint>
 #include <memory>
 #include <mutex>
@@ -134,9 +133,5 @@ void batch_release(ent_graph *g);
 // The caller holds g->mu.
 int copy_to_host(ent_graph *g, void *dst, const void *src, size_t bytes);
 
-// One single-source search from root into the device array depth, with
-// g->mu held and g->ev0 already recorded on g's stream; fills *st (may be
-// NULL). t0 starts the host window of st->host_ms (bfs_plain.hip).
-int search_locked(ent_graph *g, int root, int *depth, ent_stats *st, std::chrono::steady_clock::time_point t0);
 
 }  // namespace ent
