@@ -306,3 +306,21 @@ def test_batch_grid_many_flushes(oracle, launch):
         reached = out >= 0
         assert st["reached_vertices"] == int(reached.sum())
         assert st["reached_edges"] == int((reached * deg[None, :]).sum())
+
+
+@pytest.mark.parametrize("launch", ["stream", "persistent"])
+def test_batch_flushes_wide_masks(oracle, launch):
+    """Flushes with eight mask words per vertex: 300 sources (W = 8) on a
+    100 x 30 grid keeping 8 levels; every row against the oracle."""
+    form = {"stream": L.LAUNCH_STREAM, "persistent": L.LAUNCH_PERSISTENT}[launch]
+    g = HostGraph.grid(100, 30, delete_ppm=20000, seed=300, keep_lcc=True)
+    roots = g.pick_roots(300, 301)
+    with DeviceGraph(g) as dg:
+        dg.configure(launch=form)
+        dg.batch_reserve(300, max_levels=8)
+        _say(f"[gpu] batch grid100x30 W=8 {launch}")
+        out, st = dg.bfs_batch(roots)
+        assert st["words"] == 8
+        for i, r in enumerate(roots):
+            assert np.array_equal(out[i], oracle.bfs(g.beg_pos, g.adj, int(r))), (i, int(r))
+        assert st["flushes"] == expected_flushes(st["levels"], 8) > 5
