@@ -785,15 +785,18 @@ __device__ __forceinline__ unsigned nibble_bytes(unsigned long long m, int k) {
 // rows of the group with 256-byte coalesced writes. A vertex's mask line of
 // level L is fetched only when the vertex is in that level's joint frontier
 // (about 2 of 8 levels at R-MAT scale 23, none for a vertex without edges).
-// The wavefronts of a block share one segment (wavefront w takes groups w,
-// w + 4, ...), so each fetched line is read by every group while hot.
+// With W > 1 the wavefronts of a block share one segment (wavefront w takes
+// groups w, w + 4, ...), so each fetched line is read by every group while
+// hot; with one group, the block claims four segments at once, one per
+// wavefront.
 // Blocks claim segments from a counter of the launch (work): the grid holds
 // more blocks than are resident at once, and blocks that start late take
 // fewer segments instead of an equal share.
 // Levels lo .. hi - 1 are replayed; the table holds L - lo. With lo = 0 the
 // rows are written whole (-1 where unreached); with lo > 0 earlier flushes
-// wrote levels below lo, so only the depths found here are stored. The final
-// pass (not a flush) also counts the reached vertices and edges.
+// wrote levels below lo, so only the depths found here are stored, one row
+// per step with a lane per vertex. The final pass (not a flush) also counts
+// the reached vertices and edges.
 template <int BS, int W>
 __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, bool final,
                               unsigned long long *work) {
@@ -810,15 +813,20 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, boo
     const int t = s0 + (L - lo);
     return t >= a.ring ? t - a.ring : t;
   };
+  // Segments per claim and this wavefront's segment and first group.
+  constexpr int SPC = W == 1 ? NWV : 1;
+  const int wseg = W == 1 ? wid : 0;
+  const int g0 = W == 1 ? 0 : wid;
   for (;;) {
-    if (threadIdx.x == 0) sm.seg = (long long)atomicAdd(work, 1ull);
+    if (threadIdx.x == 0) sm.seg = (long long)atomicAdd(work, (unsigned long long)SPC);
     __syncthreads();
-    const long long wd = sm.seg;
+    const long long wd = sm.seg + wseg;
     __syncthreads();
-    if (wd >= a.nwords) break;
+    if (wd - wseg >= a.nwords) break;
+    if (wd >= a.nwords) continue;
     const long long v = wd * 64 + lane;
     const bool valid = v < a.n;
-    if (final && valid && wid == 0) {
+    if (final && valid && g0 == 0) {
       const Mask<W> sd = load_mask<W>(a.seen, v);
       int pc = 0;
 #pragma unroll
@@ -826,7 +834,7 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, boo
       verts += (unsigned long long)pc;
       edges += (unsigned long long)pc * (unsigned long long)(a.off[v + 1] - a.off[v]);
     }
-    for (int g = wid; g < W; g += NWV) {
+    for (int g = g0; g < W; g += NWV) {
       unsigned int *row = reinterpret_cast<unsigned int *>(d8 + lane * 68);
 #pragma unroll
       for (int j = 0; j < 16; ++j) row[j] = 0xFFFFFFFFu;
@@ -881,7 +889,7 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, boo
       }
       wave_sync();
       const int nb = min(64, a.k - 64 * g);
-      if (a.out16) {
+      if (a.out16 && lo == 0) {
         // 16-byte streaming stores: lane l writes vertices 4(l%16) .. +3 of
         // the segment for source 4j + l/16, j = 0..15 (every row starts on a
         // 16-byte boundary: out is 16-byte aligned and n a multiple of 4).
@@ -895,19 +903,12 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, boo
             const unsigned char d0 = d8[(4 * q + 0) * 68 + b], d1 = d8[(4 * q + 1) * 68 + b];
             const unsigned char d2 = d8[(4 * q + 2) * 68 + b], d3 = d8[(4 * q + 3) * 68 + b];
             int *dst = a.out + (long long)(64 * g + b) * a.n + v0;
-            if (lo == 0) {
-              v4i x;
-              x.x = d0 == 0xFF ? -1 : int(d0);
-              x.y = d1 == 0xFF ? -1 : int(d1);
-              x.z = d2 == 0xFF ? -1 : int(d2);
-              x.w = d3 == 0xFF ? -1 : int(d3);
-              __builtin_nontemporal_store(x, reinterpret_cast<v4i *>(dst));
-            } else {
-              if (d0 != 0xFF) dst[0] = lo + int(d0);
-              if (d1 != 0xFF) dst[1] = lo + int(d1);
-              if (d2 != 0xFF) dst[2] = lo + int(d2);
-              if (d3 != 0xFF) dst[3] = lo + int(d3);
-            }
+            v4i x;
+            x.x = d0 == 0xFF ? -1 : int(d0);
+            x.y = d1 == 0xFF ? -1 : int(d1);
+            x.z = d2 == 0xFF ? -1 : int(d2);
+            x.w = d3 == 0xFF ? -1 : int(d3);
+            __builtin_nontemporal_store(x, reinterpret_cast<v4i *>(dst));
           }
         }
       } else if (valid) {
