@@ -55,7 +55,8 @@ class BfsConfig(C.Structure):
     _fields_ = [(name, C.c_int32) for name in (
         "direction", "launch", "alpha", "beta", "t_wave", "t_block", "t_grid",
         "piece", "block_size", "blocks_per_cu", "bu_probe", "bu_chunk_words",
-        "graph_levels", "queue_small", "batch_alpha", "grid_blocks", "hub_cache", "hub_td", "queue_max_degree", "count_examined")]
+        "graph_levels", "queue_small", "batch_alpha", "grid_blocks", "hub_cache", "hub_td", "queue_max_degree", "count_examined",
+        "batch_fused")]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -54,7 +54,7 @@
 
 namespace ent {
 
-enum BPhase : int { BPH_TD = 0, BPH_BU = 1, BPH_APPLY = 2, BPH_OUTPUT = 3 };
+enum BPhase : int { BPH_TD = 0, BPH_BU = 1, BPH_APPLY = 2, BPH_OUTPUT = 3, BPH_TDF = 4 };
 enum : int { BERR_TIMEOUT = 16 };
 
 constexpr int kBStripes = 64;
@@ -102,6 +102,10 @@ struct BState {
   int resume;     // phase to run after the flush
   int flushes;    // flushes so far
   int slot;       // ring slot of `level` in lv and jf
+  // Degree sum of the vertices still missing an active source, as last
+  // counted (a fused level does not sweep the vertices; the sum only falls,
+  // so the last count bounds it from above).
+  unsigned long long munf_last;
 };
 
 struct BControl {
@@ -136,6 +140,10 @@ struct BArgs {
   int probe;
   int chunk_words;
   int out16;  // out is 16-byte aligned and n % 4 == 0: rows in 16-byte stores
+  // Largest joint-frontier degree sum expanded as a fused level (0 = never;
+  // W <= 4 only). When nonzero, every level also empties the ring slot two
+  // levels ahead (see prep_slot).
+  long long fused;
 };
 
 template <int W>
@@ -316,7 +324,10 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
     return s;
   }
   for (int w = 0; w < 8; ++w) s.active[w] = w < W ? tot[C_ACTIVE + w] : 0ull;
-  const unsigned long long mf = tot[C_MF], munf = tot[C_MUNF];
+  // A fused level counts no unfinished vertices: its successor steers with
+  // the last count.
+  const unsigned long long mf = tot[C_MF], munf = p.phase == BPH_TDF ? p.munf_last : tot[C_MUNF];
+  s.munf_last = munf;
   // Direction per 64-source word: a word whose new frontier is large is
   // pulled, a word that lags is pushed, so a pull can stop once the large
   // words are covered instead of scanning every row for the lagging ones.
@@ -340,6 +351,8 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
   s.pull_words = pull;
   s.push_words = live & ~pull;
   s.phase = s.push_words ? BPH_TD : BPH_BU;
+  // A small all-push frontier is pushed and settled in one launch.
+  if (pull == 0 && a.fused > 0 && mf <= (unsigned long long)a.fused) s.phase = BPH_TDF;
   // Level L' = s.level uses the slots of L' (its frontier), L' + 1 (its new
   // bits) and L' + 2 (cleared for the next push). The ring holds them beside
   // the levels base .. L' - 1 not yet in the rows while L' - base <= lcap;
@@ -410,27 +423,106 @@ __device__ __forceinline__ bool push_mask(Mask<W> &f, unsigned words) {
   return any;
 }
 
+// Per-lane counts of a fused level's claims (the APPLY counts, taken where
+// bits first land instead of in a sweep over every vertex).
 template <int W>
-__device__ __forceinline__ void push_edge(const BArgs &a, unsigned long long *dst, int u, const Mask<W> &f) {
+struct Claims {
+  unsigned long long jw, mf, alg;
+  unsigned long long mfw[W];
+  Mask<W> act;
+};
+
+// One edge of the push to u. Plain: OR the bits of f that u has not seen into
+// lv[L + 1][u], fire and forget. CLAIM (a fused level): the bits also go into
+// seen[u] at once, and the OR into lv[L + 1][u] returns the word's old value.
+// That word was zero before the level began (prep_slot), so the edge that
+// finds it zero is the first to reach u in that word: it counts u's word, and
+// the first word to claim u also sets u's bit in the new joint frontier jn.
+template <int W, bool CLAIM>
+__device__ __forceinline__ void push_edge(const BArgs &a, unsigned long long *dst, int u, const Mask<W> &f,
+                                          unsigned long long *jn, Claims<W> &cl) {
   const Mask<W> sd = load_mask<W>(a.seen, u);
+  if (!CLAIM) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const unsigned long long nb = f.w[w] & ~sd.w[w];
+      if (nb) atomicOr(dst + (long long)u * W + w, nb);
+    }
+    return;
+  }
+  unsigned long long old[W];
+  bool first = false;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     const unsigned long long nb = f.w[w] & ~sd.w[w];
-    if (nb) atomicOr(dst + (long long)u * W + w, nb);
+    old[w] = ~0ull;
+    if (nb) {
+      atomicOr(a.seen + (long long)u * W + w, nb);
+      old[w] = atomicOr(dst + (long long)u * W + w, nb);
+      cl.act.w[w] |= nb;
+      first |= old[w] == 0;
+    }
+  }
+  if (!first) return;
+  const long long deg = a.off[u + 1] - a.off[u];
+  const unsigned long long dc = (unsigned long long)min(deg, (long long)(1 << 25));
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    if (old[w] == 0) {
+      cl.jw += 1;
+      cl.alg += 8ull + 12ull * (unsigned long long)deg;
+      cl.mfw[w] += dc;
+    }
+  }
+  const unsigned long long bit = 1ull << (u & 63);
+  if ((atomicOr(jn + (u >> 6), bit) & bit) == 0) cl.mf += (unsigned long long)deg;
+}
+
+template <int W, bool CLAIM>
+__device__ __forceinline__ void push_range(const BArgs &a, unsigned long long *dst, long long s, long long e,
+                                           int t, int stride, const Mask<W> &f, unsigned long long *jn,
+                                           Claims<W> &cl) {
+  for (long long i = s + t; i < e; i += stride) push_edge<W, CLAIM>(a, dst, a.adj[i], f, jn, cl);
+}
+
+// Empties ring slot `slot` for the level that will next use it: clears the
+// mask of every vertex in the slot's old joint frontier, then the bitmap.
+// Every nonzero mask of a slot belongs to a vertex in that slot's bitmap (a
+// push reaches only vertices that join the new frontier, a pull stores only
+// new bits), so the slot is then zero throughout, as a fused level's claims
+// require. Thread per bitmap word.
+template <int BS, int W>
+__device__ void prep_slot(const BArgs &a, int slot) {
+  unsigned long long *lz = level_ptr(a, slot);
+  unsigned long long *jz = jf_ptr(a, slot);
+  const long long nt = (long long)gridDim.x * BS;
+  for (long long wd = (long long)blockIdx.x * BS + threadIdx.x; wd < a.nwords; wd += nt) {
+    unsigned long long m = jz[wd];
+    if (!m) continue;
+    jz[wd] = 0;
+    while (m) {
+      const long long v = wd * 64 + (__ffsll((long long)m) - 1);
+      m &= m - 1;
+#pragma unroll
+      for (int w = 0; w < W; ++w) lz[v * W + w] = 0;
+    }
   }
 }
 
-template <int W>
-__device__ __forceinline__ void push_range(const BArgs &a, unsigned long long *dst, long long s, long long e,
-                                           int t, int stride, const Mask<W> &f) {
-  for (long long i = s + t; i < e; i += stride) push_edge<W>(a, dst, a.adj[i], f);
-}
-
-template <int BS, int W>
+// Top-down level L. Plain (TD): pushes only; APPLY or the pull settles the
+// new bits. CLAIM (TDF, a fused level): pushes with claims, so the new
+// frontier's bitmap, seen and the counts are complete when the launch ends,
+// and empties the slot of level L + 2 (prep_slot) as APPLY and the pull do.
+template <int BS, int W, bool CLAIM>
 __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const unsigned long long *fs = level_ptr(a, ring_slot(a, sm.S, 0));
   unsigned long long *dst = level_ptr(a, ring_slot(a, sm.S, 1));
   const unsigned long long *jf = jf_ptr(a, ring_slot(a, sm.S, 0));
+  unsigned long long *jn = jf_ptr(a, ring_slot(a, sm.S, 1));
+  Claims<W> cl;
+  cl.jw = cl.mf = cl.alg = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) cl.mfw[w] = cl.act.w[w] = 0;
   const unsigned pw = sm.S.push_words;  // only these words are pushed at this level
   for (int p = blockIdx.x; p < a.n_pieces; p += gridDim.x) {
     const int v = a.piece_v[p];
@@ -438,14 +530,14 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     Mask<W> f = load_mask<W>(fs, v);
     if (!push_mask<W>(f, pw)) continue;
     const long long s = a.off[v] + (long long)a.piece_k[p] * a.piece;
-    push_range<W>(a, dst, s, min(a.off[v + 1], s + a.piece), threadIdx.x, BS, f);
+    push_range<W, CLAIM>(a, dst, s, min(a.off[v + 1], s + a.piece), threadIdx.x, BS, f, jn, cl);
   }
   for (int h = blockIdx.x; h < a.n_hub_block; h += gridDim.x) {
     const int v = a.hub_block[h];
     if (!bitv(jf, v)) continue;
     Mask<W> f = load_mask<W>(fs, v);
     if (!push_mask<W>(f, pw)) continue;
-    push_range<W>(a, dst, a.off[v], a.off[v + 1], threadIdx.x, BS, f);
+    push_range<W, CLAIM>(a, dst, a.off[v], a.off[v + 1], threadIdx.x, BS, f, jn, cl);
   }
   const int wid = threadIdx.x / kWave, lane = lane_id();
   unsigned long long *wc = sm.cand[wid];
@@ -479,7 +571,7 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
         e = a.off[v + 1];
         if (e - s < a.t_wave) {
           Mask<W> f = load_mask<W>(fs, v);
-          if (push_mask<W>(f, pw)) push_range<W>(a, dst, s, e, 0, 1, f);
+          if (push_mask<W>(f, pw)) push_range<W, CLAIM>(a, dst, s, e, 0, 1, f, jn, cl);
         }
       }
       const long long d = e - s;
@@ -490,11 +582,37 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
         const int vv = __shfl(v, src);
         const long long ss = __shfl(s, src), ee = __shfl(e, src);
         Mask<W> f = load_mask<W>(fs, vv);
-        if (push_mask<W>(f, pw)) push_range<W>(a, dst, ss, ee, lane, kWave, f);
+        if (push_mask<W>(f, pw)) push_range<W, CLAIM>(a, dst, ss, ee, lane, kWave, f, jn, cl);
       }
     }
     wave_sync();
   }
+  if (CLAIM) {
+    prep_slot<BS, W>(a, ring_slot(a, sm.S, 2));
+    WordSums<W> mfw;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      unsigned long long x = cl.mfw[w];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+      mfw.d[w] = x;
+    }
+    wave_counts<BS, W>(sm, cl.jw, cl.mf, cl.alg, 0, cl.act, 0, 0, mfw);
+  }
+}
+
+// prep_slot for one bitmap word inside a sweep with a lane per vertex: the
+// masks of the word's old members (bits of oj, wave-uniform), then the word.
+template <int W>
+__device__ __forceinline__ void clear_old_members(unsigned long long *lz, unsigned long long *jz, long long wd,
+                                                  unsigned long long oj, int lane) {
+  if ((oj >> lane) & 1ull) {
+    Mask<W> z;
+#pragma unroll
+    for (int w = 0; w < W; ++w) z.w[w] = 0;
+    store_mask<W>(lz, wd * 64 + lane, z);
+  }
+  if (lane == 0) jz[wd] = 0;
 }
 
 // Tail of APPLY for the vertex of this lane: new bits x (the lane's vertex
@@ -544,6 +662,8 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   unsigned long long *nx = level_ptr(a, ring_slot(a, sm.S, 1));
   unsigned long long *jn = jf_ptr(a, ring_slot(a, sm.S, 1));
   unsigned long long *clr = level_ptr(a, ring_slot(a, sm.S, 2));  // lv[L + 2]
+  unsigned long long *pj = jf_ptr(a, ring_slot(a, sm.S, 2));
+  const bool prep = W <= 4 && a.fused > 0;  // empty the slot of level L + 2 (prep_slot)
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
@@ -554,6 +674,7 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   for (long long wd = (long long)blockIdx.x * (BS / kWave) + threadIdx.x / kWave; wd < a.nwords; wd += nwv) {
     const long long v = wd * 64 + lane;
     const bool valid = v < a.n;
+    const unsigned long long oj = prep ? pj[wd] : 0ull;
     Mask<W> x, sd;
 #pragma unroll
     for (int w = 0; w < W; ++w) x.w[w] = sd.w[w] = 0;
@@ -578,6 +699,7 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     settle_vertex<W>(a, clr, valid, v, sd, x, deg, sm.S.active, jw, mf, alg, munf, act, mfw, any);
     const unsigned long long bits = __ballot(any);
     if (lane == 0) jn[wd] = bits;
+    if (oj) clear_old_members<W>(clr, pj, wd, oj, lane);
   }
   wave_counts<BS, W>(sm, jw, mf, alg, munf, act, 0, 0, mfw);
 }
@@ -642,6 +764,8 @@ __device__ void bbu_phase_h(const BArgs &a, BSmem<BS> &sm, int L) {
   unsigned long long *nx = level_ptr(a, ring_slot(a, sm.S, 1));
   unsigned long long *jn = jf_ptr(a, ring_slot(a, sm.S, 1));
   unsigned long long *clr = level_ptr(a, ring_slot(a, sm.S, 2));  // lv[L + 2]
+  unsigned long long *pj = jf_ptr(a, ring_slot(a, sm.S, 2));
+  const bool prep = W <= 4 && a.fused > 0;  // empty the slot of level L + 2 (prep_slot)
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
@@ -656,6 +780,7 @@ __device__ void bbu_phase_h(const BArgs &a, BSmem<BS> &sm, int L) {
     // A vertex without edges can neither pull nor be pushed to (the pull
     // runs only on a symmetric adjacency): nothing of it is loaded.
     const bool valid = v < a.n && ((a.nz[wd] >> lane) & 1ull) != 0;
+    const unsigned long long oj = prep ? pj[wd] : 0ull;
     Mask<W> tgt, acc;
     bool need = false;
     int ftc = 0;
@@ -759,6 +884,7 @@ __device__ void bbu_phase_h(const BArgs &a, BSmem<BS> &sm, int L) {
     settle_bu<W>(a, clr, valid, v, ftc, x, e - s, jw, mf, alg, munf, act, mfw, any);
     const unsigned long long bits = __ballot(any);
     if (lane == 0) jn[wd] = bits;
+    if (oj) clear_old_members<W>(clr, pj, wd, oj, lane);
   }
   wave_counts<BS, W>(sm, jw, mf, alg, munf, act, 0, 0, mfw);
 }
@@ -940,7 +1066,10 @@ __global__ __launch_bounds__(BS) void k_blevel(BArgs a, long long i) {
   if (S.done || S.phase == BPH_BU) return;
   const int slot = int(i % 3);
   switch (S.phase) {
-    case BPH_TD: btd_phase<BS, W>(a, sm, S.level); break;
+    case BPH_TD: btd_phase<BS, W, false>(a, sm, S.level); break;
+    case BPH_TDF:
+      if (W <= 4) btd_phase<BS, W, true>(a, sm, S.level);
+      break;
     case BPH_APPLY: bapply_phase<BS, W>(a, sm, S.level); break;
     default:
       boutput_phase<BS, W>(a, sm, S.base, S.flush ? S.flush_end : S.levels, !S.flush, &a.ctl->cnt[slot].v[C_WORK]);
@@ -980,7 +1109,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(bpersist_wav
     if (S.done || (S.error & BERR_TIMEOUT)) break;
     const int slot = int(i % 3);
     switch (S.phase) {
-      case BPH_TD: btd_phase<BS, W>(a, sm, S.level); break;
+      case BPH_TD: btd_phase<BS, W, false>(a, sm, S.level); break;
+      case BPH_TDF:
+        if (W <= 4) btd_phase<BS, W, true>(a, sm, S.level);
+        break;
       case BPH_APPLY: bapply_phase<BS, W>(a, sm, S.level); break;
       case BPH_BU: bbu_phase<BS, W>(a, sm, S.level); break;
       default:
@@ -1004,7 +1136,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(bpersist_wav
 
 // Zeroes seen, the level-0 joint frontier, lv[0] at the roots (the only
 // vertices where it is read) and lv[1] where level 0 can push; clears the
-// counters and the grid barrier; sets the pseudo-level -1 state.
+// counters and the grid barrier; sets the pseudo-level -1 state. With fused
+// levels, also empties the slots of levels 0 and 1 whole (prep_slot; the
+// level-1 masks are cleared at every vertex a push can reach).
 __global__ void k_binit_zero(BArgs a, const int *roots) {
   const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long nt = (long long)gridDim.x * blockDim.x;
@@ -1017,7 +1151,16 @@ __global__ void k_binit_zero(BArgs a, const int *roots) {
     if (receives(a, a.off[v + 1] - a.off[v])) l1[i] = 0;
   }
   for (long long i = gt; i < (long long)a.k * W; i += nt) a.lv[(long long)roots[i / W] * W + i % W] = 0;
-  for (long long w = gt; w < a.nwords; w += nt) a.jf[w] = 0;
+  for (long long w = gt; w < a.nwords; w += nt) {
+    if (a.fused > 0) {
+      for (unsigned long long m = a.jf[w]; m; m &= m - 1) {
+        const long long v = w * 64 + (__ffsll((long long)m) - 1);
+        for (int j = 0; j < W; ++j) a.lv[v * W + j] = 0;
+      }
+      a.jf[a.nwords + w] = 0;
+    }
+    a.jf[w] = 0;
+  }
   constexpr long long kC = (long long)kBStripes * kBStripeU64;
   for (long long i = gt; i < 3 * kC; i += nt) a.ctl->cnt[i / kC].v[i % kC] = 0;
   for (long long i = gt; i < kSyncStripes * kSyncStride; i += nt) a.ctl->sync.arrive[i] = 0;
@@ -1113,6 +1256,11 @@ struct BatchWorkspace {
   int expect_launches = 16;
   int *rows = nullptr;     // device rows of ent_bfs_batch (grown on demand)
   size_t rows_bytes = 0;
+  // Every nonzero mask in the lv ring belongs to a vertex in its slot's
+  // bitmap (what fused levels rely on). Holds after the rings are zeroed and
+  // after a search that completed with fused levels allowed; a search that
+  // did not, or failed, may leave masks the bitmaps do not list.
+  bool rings_clean = false;
 };
 
 }  // namespace ent
@@ -1219,6 +1367,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   a.probe = c.bu_probe;
   a.chunk_words = c.bu_chunk_words;
   a.out16 = ((reinterpret_cast<uintptr_t>(out) & 15) == 0 && (g->n & 3) == 0) ? 1 : 0;
+  a.fused = (W <= 4 && c.batch_fused > 0) ? std::max<long long>(1, (long long)g->n / c.batch_fused) : 0;
   // Level-0 counts on the host: distinct (vertex, word) pairs of the roots.
   std::vector<std::pair<int32_t, int>> vw;
   std::vector<int32_t> vs;
@@ -1241,6 +1390,14 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   init.munf = (unsigned long long)g->m;  // every vertex still needs sources
   HIP_TRY(hipMemcpyAsync(b->roots, roots, sizeof(int32_t) * size_t(k), hipMemcpyHostToDevice, g->stream));
   const int grid = g->cus * 8;
+  const bool was_clean = b->rings_clean;
+  b->rings_clean = false;  // until this search completes
+  if (a.fused > 0 && !was_clean) {
+    const long long nlv = a.nW * (long long)a.ring, njf = (long long)a.nwords * (long long)a.ring;
+    hipLaunchKernelGGL(k_zero_u64, dim3(grid), dim3(256), 0, g->stream, b->lv, nlv);
+    hipLaunchKernelGGL(k_zero_u64, dim3(grid), dim3(256), 0, g->stream, b->jf, njf);
+    HIP_TRY(hipGetLastError());
+  }
   {
     void *args[] = {&a, &b->roots};
     HIP_TRY(hipLaunchKernel(reinterpret_cast<const void *>(&k_binit_zero), dim3(grid), dim3(256), args, 0, g->stream));
@@ -1272,6 +1429,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
     *fin = s0.phases >= s1.phases ? s0 : s1;
     fin->error |= (s0.error | s1.error) & BERR_TIMEOUT;
     g->last_grid = pgrid;
+    b->rings_clean = a.fused > 0 && fin->done && !fin->error;
     return ENT_OK;
   }
   const int lgrid = c.grid_blocks > 0 ? c.grid_blocks : g->cus * std::max(1, c.blocks_per_cu > 0 ? c.blocks_per_cu : 8);
@@ -1295,6 +1453,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   }
   b->expect_launches = std::min(fin->phases + 1, 512);
   g->last_grid = lgrid;
+  b->rings_clean = a.fused > 0 && !fin->error;
   return ENT_OK;
 }
 

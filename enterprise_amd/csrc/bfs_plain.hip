@@ -1252,6 +1252,7 @@ int check_config(const ent_bfs_config *c) {
   if (c->queue_max_degree < 0) return set_error(ENT_EINVAL, "queue_max_degree %d", c->queue_max_degree);
   if (c->count_examined != 0 && c->count_examined != 1)
     return set_error(ENT_EINVAL, "count_examined %d not 0 or 1", c->count_examined);
+  if (c->batch_fused < 0 || c->batch_fused > (1 << 20)) return set_error(ENT_EINVAL, "batch_fused %d", c->batch_fused);
   return ENT_OK;
 }
 
@@ -1634,6 +1635,7 @@ void ent_bfs_default_config(ent_bfs_config *c) {
   c->hub_td = 0;
   c->queue_max_degree = 0;
   c->count_examined = 0;
+  c->batch_fused = 8;
 }
 
 int ent_graph_upload(const int64_t *beg_pos, const int32_t *adj, int64_t n, int64_t m, int device,

@@ -94,6 +94,13 @@ typedef struct ent_bfs_config {
   int32_t count_examined;   /* 1: run the kernel variant that counts the
                                work it examines (ent_stats.examined_*);
                                for measurement, not speed */
+  int32_t batch_fused;      /* multi-source search with at most 256 sources
+                               per workspace (ent_batch.h): a level that
+                               pushes every source word and whose joint
+                               frontier's degree sum is at most
+                               n / batch_fused is pushed and settled in one
+                               launch, with no sweep over every vertex
+                               (0 = never) */
 } ent_bfs_config;
 
 typedef struct ent_stats {

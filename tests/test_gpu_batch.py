@@ -324,3 +324,80 @@ def test_batch_flushes_wide_masks(oracle, launch):
         for i, r in enumerate(roots):
             assert np.array_equal(out[i], oracle.bfs(g.beg_pos, g.adj, int(r))), (i, int(r))
         assert st["flushes"] == expected_flushes(st["levels"], 8) > 5
+
+
+def _rows_and_stats(dg, roots, **cfg):
+    base = dg.config
+    try:
+        dg.configure(**cfg)
+        return dg.bfs_batch(roots)
+    finally:
+        dg.set_config(base)
+
+
+@pytest.mark.parametrize("launch", ["stream", "persistent"])
+def test_batch_fused_levels(oracle, rmat16, launch):
+    """Fused levels (push and settle in one launch, claims by the returning
+    OR into the next level's masks) give the rows of unfused levels. One
+    workspace per source count (W = 1, 2, 4) runs batch_fused = 0, 1 (every
+    all-push level fused), 8, 0, 1 in a row, so searches that leave the
+    level ring unlisted are followed by fused ones; every row equals the
+    unfused row, the first rows the oracle, and the counts agree."""
+    form = {"stream": L.LAUNCH_STREAM, "persistent": L.LAUNCH_PERSISTENT}[launch]
+    g, dg = rmat16
+    for k in (64, 128, 256):
+        dg.batch_release()  # a workspace of the batch's own width
+        roots = g.pick_roots_mixed(k, k // 4, 4000 + k)
+        ref = None
+        for fused in (0, 1, 8, 0, 1):
+            _say(f"[gpu] batch fused={fused} k={k} {launch}")
+            out, st = _rows_and_stats(dg, roots, launch=form, batch_fused=fused)
+            assert st["words"] == k // 64
+            if ref is None:
+                ref = (out, st)
+                for i in range(4):
+                    assert np.array_equal(out[i], oracle.bfs(g.beg_pos, g.adj, int(roots[i])))
+                continue
+            assert np.array_equal(out, ref[0]), (k, fused)
+            for key in ("levels", "reached_vertices", "reached_edges", "alg_bytes"):
+                assert st[key] == ref[1][key], (k, fused, key)
+        dg.batch_release()
+
+
+@pytest.mark.parametrize("launch", ["stream", "persistent"])
+def test_batch_fused_grid(oracle, launch):
+    """A high-diameter batch with fused levels takes one launch per such
+    level instead of two, with flushes in between (200 x 40 grid, 64
+    sources, 16 levels kept); rows against the oracle."""
+    form = {"stream": L.LAUNCH_STREAM, "persistent": L.LAUNCH_PERSISTENT}[launch]
+    g = HostGraph.grid(200, 40, delete_ppm=20000, seed=240, keep_lcc=True)
+    roots = g.pick_roots(64, 242)
+    with DeviceGraph(g) as dg:
+        dg.batch_reserve(64, max_levels=16)
+        _say(f"[gpu] batch fused grid200x40 {launch}")
+        plain, st0 = _rows_and_stats(dg, roots, launch=form, batch_fused=0)
+        fused, st1 = _rows_and_stats(dg, roots, launch=form, batch_fused=1)
+        for i, r in enumerate(roots):
+            assert np.array_equal(fused[i], oracle.bfs(g.beg_pos, g.adj, int(r))), (i, int(r))
+        assert np.array_equal(plain, fused)
+        assert st1["levels"] == st0["levels"] > 150
+        assert st1["flushes"] == st0["flushes"] > 10
+        # Levels whose joint frontier's degree sum exceeds n stay unfused.
+        assert st1["phases"] < 0.75 * st0["phases"], (st0["phases"], st1["phases"])
+        assert st1["alg_bytes"] == st0["alg_bytes"]
+
+
+def test_batch_fused_directed_topdown(oracle):
+    """Fused forced top-down levels on a directed adjacency: vertices without
+    out-edges are claimed through their in-edges like any other."""
+    beg, adj = graphs.directed_with_sinks(3000, 1000, 3, 79)
+    rng = np.random.default_rng(80)
+    with DeviceGraph(beg_pos=beg, adj=adj) as dg:
+        dg.configure(direction=L.DIR_TOPDOWN, batch_fused=1)
+        for k in (200, 64):
+            roots = rng.choice(3000, size=k, replace=False).astype(np.int32)
+            _say(f"[gpu] batch fused directed/topdown k={k}")
+            out, st = dg.bfs_batch(roots)
+            for i, r in enumerate(roots):
+                assert np.array_equal(out[i], oracle.bfs(beg, adj, int(r))), (k, i, int(r))
+            assert np.any(out[:, 3000:] > 0)
