@@ -112,6 +112,7 @@ struct BControl {
   BState st[2];
   GridSync sync;  // persistent form only
   BCounters cnt[3];
+  int lphase[2];  // phase of launch i in lphase[i & 1] (-1 once done)
 };
 
 struct BArgs {
@@ -400,7 +401,10 @@ __device__ __forceinline__ void begin_blaunch(const BArgs &a, BSmem<BS> &sm, lon
       }
       const BState S = next_bstate(a, prev, t, W);
       sm.S = S;
-      if (blockIdx.x == 0) c->st[i & 1] = S;
+      if (blockIdx.x == 0) {
+        c->st[i & 1] = S;
+        c->lphase[i & 1] = S.done ? -1 : S.phase;
+      }
     }
     if (blockIdx.x == 0 && lane < C_COUNT)
       for (int s = 0; s < kBStripes; ++s) c->cnt[(i + 1) % 3].v[s * kBStripeU64 + lane] = 0;
@@ -1080,6 +1084,9 @@ __global__ __launch_bounds__(BS) void k_blevel(BArgs a, long long i) {
 
 template <int BS, int W>
 __global__ __launch_bounds__(BS) void k_blevel_bu(BArgs a, long long i) {
+  // k_blevel ran launch i just before on the same stream and recorded its
+  // phase: unless it is a pull, leave without reading the state and counters.
+  if (a.ctl->lphase[i & 1] != BPH_BU) return;
   __shared__ BSmem<BS> sm;
   begin_blaunch<BS, W>(a, sm, i);
   const BState &S = sm.S;
