@@ -190,16 +190,17 @@ def grid(args):
         dg.bfs_device(int(r))
         violations += sum(dg.validate_device(int(r)))
     best = min(res, key=lambda k: res[k]["ms_per_bfs"])
-    # The same graph as one 64-source batch keeping 48 levels (the level ring
-    # flushes to the rows about every 49 levels), against 64 searches at the
-    # best form's mean time; four rows checked against the single-source path.
+    # The same graph as one 64-source batch keeping 250 levels (the most the
+    # level ring holds: it flushes to the rows about every 251 levels),
+    # against 64 searches at the best form's mean time; four rows checked
+    # against the single-source path.
     batch = None
     if not getattr(args, "no_grid_batch", False):
         k = 64
         broots = g.pick_roots(k, 1007)
         out = dg.device_alloc(k * g.n * 4)
         try:
-            dg.batch_reserve(k, 48)
+            dg.batch_reserve(k, 250)
             t1 = time.perf_counter()
             st = dg.bfs_batch_device(broots, out)
             bms = (time.perf_counter() - t1) * 1e3
@@ -212,7 +213,7 @@ def grid(args):
             dg.device_free(out)
             dg.batch_release()
         batch = {"k": k, "roots_seed": 1007, "batch_ms": round(bms, 1), "levels": st["levels"],
-                 "flushes": st["flushes"], "kept_levels": 48,
+                 "flushes": st["flushes"], "kept_levels": 250,
                  "speedup_vs_plain": round(k * res[best]["ms_per_bfs"] / bms, 2),
                  "rows_checked": 4, "rows_mismatching_plain": mism}
         log(f"[grid] batch of {k}: {batch}")

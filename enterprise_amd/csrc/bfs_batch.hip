@@ -546,7 +546,9 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const int wid = threadIdx.x / kWave, lane = lane_id();
   unsigned long long *wc = sm.cand[wid];
   int *we = sm.excl[wid];
-  const int CW = a.chunk_words;
+  // A fused level's frontier is sparse: chunks of 64 bitmap words (4,096
+  // vertices) keep a wavefront's lanes busy and its chunks per level few.
+  const int CW = CLAIM ? kWave : a.chunk_words;
   const long long nchunks = ((long long)a.nwords + CW - 1) / CW;
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   for (long long ch = (long long)blockIdx.x * (BS / kWave) + wid; ch < nchunks; ch += nwv) {
