@@ -905,11 +905,75 @@ __device__ void bbu_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   else bbu_phase_h<BS, W, -1>(a, sm, L);
 }
 
+// Transpose of a 64 x 64 bit matrix held one row per lane: returns, in lane
+// c, the word whose bit r is bit c of lane r's word (six exchange steps of
+// halving blocks).
+__device__ __forceinline__ unsigned long long lane_transpose(unsigned long long x, int lane) {
+  const unsigned long long m[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                   0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int j = 32 >> s;
+    const unsigned long long o = __shfl_xor(x, j);
+    x = (lane & j) ? (x & ~m[s]) | ((o >> j) & m[s]) : (x & m[s]) | ((o << j) & ~m[s]);
+  }
+  return x;
+}
+
 // 0xFF in byte j of the result for each set bit j of bits 4k .. 4k+3 of m.
 __device__ __forceinline__ unsigned nibble_bytes(unsigned long long m, int k) {
   unsigned x = (unsigned)(m >> (4 * k)) & 0xFu;
   x = (x | (x << 7) | (x << 14) | (x << 21)) & 0x01010101u;
   return x * 0xFFu;
+}
+
+// Output replay with one 64-source group (W = 1) of levels L0 .. L0 + 63
+// (fl: lane j holds the segment's bitmap word of level L0 + j). Each lane
+// takes its own vertex's member levels from the transposed words and loads
+// their masks two at a time; where every lane's masks are sparse (at most
+// four sources each, as on a high-diameter graph) the table gets one byte
+// per source bit, otherwise the nibble expansion.
+template <class SlotFn>
+__device__ __forceinline__ void replay_lanes(const BArgs &a, unsigned char *d8, unsigned long long fl, int L0, int lo,
+                                             long long v, SlotFn slot) {
+  const int lane = lane_id();
+  unsigned long long mem = lane_transpose(fl, lane);  // bit j: v is in level L0 + j
+  unsigned int *row = reinterpret_cast<unsigned int *>(d8 + lane * 68);
+  while (__ballot(mem != 0)) {
+    constexpr int kLv = 2;  // 4 costs the shared kernel a wavefront per SIMD
+    unsigned long long mk[kLv];
+    unsigned lvl[kLv];
+#pragma unroll
+    for (int t = 0; t < kLv; ++t) {
+      mk[t] = 0;
+      lvl[t] = 0;
+      if (mem) {
+        const int j = __ffsll((long long)mem) - 1;
+        mem &= mem - 1;
+        lvl[t] = (unsigned)(L0 - lo + j);
+        mk[t] = a.lv[(long long)slot(L0 + j) * a.nW + v];
+      }
+    }
+    bool dense = false;
+#pragma unroll
+    for (int t = 0; t < kLv; ++t) dense |= __popcll(mk[t]) > 4;
+    if (__ballot(dense)) {
+#pragma unroll 4
+      for (int k = 0; k < 16; ++k) {
+        unsigned w = row[k];
+#pragma unroll
+        for (int t = 0; t < kLv; ++t) {
+          const unsigned msk = nibble_bytes(mk[t], k);
+          w = (w & ~msk) | ((lvl[t] * 0x01010101u) & msk);
+        }
+        row[k] = w;
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < kLv; ++t)
+        for (unsigned long long b = mk[t]; b; b &= b - 1) d8[lane * 68 + (__ffsll((long long)b) - 1)] = (unsigned char)lvl[t];
+    }
+  }
 }
 
 // Output: per 64-vertex segment, replay the level masks of each 64-source
@@ -974,6 +1038,10 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, boo
         // Lane j holds the segment's joint-frontier word of level L0 + j.
         const unsigned long long fl = L0 + lane < hi ? a.jf[(long long)slot(L0 + lane) * a.nwords + wd] : 0ull;
         unsigned long long lm = __ballot(fl != 0);
+        if (W == 1) {
+          if (lm) replay_lanes(a, d8, fl, L0, lo, v, slot);
+          continue;
+        }
         // Up to kLv levels per round trip: their mask loads are issued
         // together, then replayed (a segment's wavefront visits about six
         // levels at R-MAT scale 23, one lane's vertex about two).
