@@ -2,7 +2,8 @@
 R-MAT scale 22 (64-root workload, tuned single-source configuration) against
 the oracle bit for bit; the 512-root multi-source search on R-MAT scale 23
 (tuned multi-source knobs) against the single-source path on every row and
-the oracle on sampled rows; the 4096 x 4096 grid in every launch form against
+the oracle on sampled rows, and batches of 64 and 256 of its roots with fused
+levels off and on; the 4096 x 4096 grid in every launch form against
 the oracle, and as one 64-source batch (over a hundred flushes of the level
 ring); a complete bipartite graph with more than 2^31 directed edges
 (int64 offsets end to end) in every launch form and a 64-source batch; and
@@ -88,6 +89,18 @@ def test_rmat23_batch_512_rows(oracle):
             _say("[gpu] rmat23 validate 512 rows on the device")
             for i, r in enumerate(roots):
                 assert dg.validate_device(int(r), out + 4 * g.n * i) == [0, 0, 0, 0], i
+            # Batches of one and four source words, where the small tail
+            # levels run fused, with fused levels off and on: the same rows.
+            for kk in (64, 256):
+                sub = np.r_[roots[:kk // 2], roots[256:256 + kk // 2]]
+                want = np.r_[plain[:kk // 2], plain[256:256 + kk // 2]]
+                dg.batch_release()
+                for fused in (0, 8):
+                    dg.configure(batch_fused=fused)
+                    _say(f"[gpu] rmat23 batch {kk} fused={fused}")
+                    st = dg.bfs_batch_device(sub, out)
+                    assert st["words"] == kk // 64
+                    assert np.array_equal(dg.row_hash(out, kk, g.n), want), (kk, fused)
         finally:
             dg.device_free(out)
 
