@@ -15,8 +15,9 @@ levels (bfs_gpu_opt.cuh:413-445).
 The default line also carries, under "extra", the 512-source multi-source
 search on R-MAT scale 23 against the plain path on the same roots
 (extra.batch) and the per-level cost of each launch form on the 4096^2
-high-diameter grid, with the same grid as one 64-source batch (extra.grid),
-each with its oracle parity and CPU figures (--no-extra skips both).
+high-diameter grid, with the same grid as one 64-source batch and its 16
+roots as one 16-source batch (extra.grid), each with its oracle parity and
+CPU figures (--no-extra skips both).
 
 Other modes (--mode): grid, batch, serve (continuous-batching serving loop)
 and csr (CSR construction on the device from the raw tuples of R-MAT scale
@@ -338,7 +339,7 @@ def bench_plain(args):
         log("[bench] extra: 4096^2 grid, launch forms")
         gr = run_extra(args, "grid", grid_roots=0.25, set=[])
         out["extra"]["grid"] = {k: gr[k] for k in (
-            "value", "unit", "best_form", "forms", "device_validation", "batch", "cpu_serial_ms_per_bfs",
+            "value", "unit", "best_form", "forms", "device_validation", "batch", "batch16", "cpu_serial_ms_per_bfs",
             "parity_roots", "parity_forms", "parity_mismatches", "config") if k in gr}
     return out
 

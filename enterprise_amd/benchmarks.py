@@ -186,21 +186,28 @@ def grid(args):
         checks.append((int(roots[j]), dg.bfs(int(roots[j]))[0], name))
     dg.configure(launch=L.LAUNCH_STREAM)
     violations = 0
+    single_hash, single_levels = [], []
     for r in roots:
-        dg.bfs_device(int(r))
+        st = dg.bfs_device(int(r))
         violations += sum(dg.validate_device(int(r)))
+        single_hash.append(int(dg.row_hash(dg.depth_device_ptr(), 1, g.n)[0]))
+        single_levels.append(st["levels"])
     best = min(res, key=lambda k: res[k]["ms_per_bfs"])
     # The same graph as one 64-source batch keeping 250 levels (the most the
     # level ring holds: it flushes to the rows about every 251 levels),
     # against 64 searches at the best form's mean time; four rows checked
-    # against the single-source path.
-    batch = None
+    # against the single-source path. Then the 16 roots above as one 16-source
+    # batch: every row against its single-source row, and the batch time per
+    # root and per level of a root (a multi-source form of the same 16
+    # searches, beside the single-source forms' latency per level).
+    batch = batch16 = None
     if not getattr(args, "no_grid_batch", False):
         k = 64
         broots = g.pick_roots(k, 1007)
         out = dg.device_alloc(k * g.n * 4)
         try:
             dg.batch_reserve(k, 250)
+            dg.bfs_batch_device(broots[:16], out)   # warm-up (first use of the level ring)
             t1 = time.perf_counter()
             st = dg.bfs_batch_device(broots, out)
             bms = (time.perf_counter() - t1) * 1e3
@@ -209,6 +216,23 @@ def grid(args):
             for i in (0, 21, 42, 63):
                 dg.bfs_device(int(broots[i]))
                 mism += int(int(dg.row_hash(dg.depth_device_ptr(), 1, g.n)[0]) != int(h[i]))
+            k16 = len(roots)
+            t1 = time.perf_counter()
+            st16 = dg.bfs_batch_device(roots, out)
+            b16 = (time.perf_counter() - t1) * 1e3
+            h16 = dg.row_hash(out, k16, g.n)
+            per_root = b16 / k16
+            batch16 = {"k": k16, "roots": "the 16 roots timed above (seed 1003)", "batch_ms": round(b16, 1),
+                       "ms_per_root": round(per_root, 3),
+                       "us_per_level_per_root": round(per_root * 1e3 / float(np.mean(single_levels)), 3),
+                       # the best form's cost per level times the 16 roots' mean depth
+                       "speedup_vs_best_form": round(res[best]["us_per_level"] * float(np.mean(single_levels))
+                                                     / (per_root * 1e3), 2),
+                       "levels": st16["levels"], "flushes": st16["flushes"],
+                       "rows_checked": k16,
+                       "rows_mismatching_single_source": int(sum(int(h16[i]) != single_hash[i]
+                                                                 for i in range(k16)))}
+            log(f"[grid] the 16 roots as one batch: {batch16}")
         finally:
             dg.device_free(out)
             dg.batch_release()
@@ -223,7 +247,7 @@ def grid(args):
             "config": {"workload": f"{side}x{side} 4-neighbour grid, 2% edges deleted (seed 4096), "
                                    "largest component, 16 roots (seed 1003)", "n": g.n, "m": g.m},
             "forms": res, "device_validation": {"roots": len(roots), "violations": int(violations)},
-            "batch": batch,
+            "batch": batch, "batch16": batch16,
             "_cpu_check": {"graph": g, "checks": checks}}
 
 
