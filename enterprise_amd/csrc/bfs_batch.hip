@@ -106,6 +106,8 @@ struct BState {
   // counted (a fused level does not sweep the vertices; the sum only falls,
   // so the last count bounds it from above).
   unsigned long long munf_last;
+  int pulled;  // some level of this search pulled
+  int pad;
 };
 
 struct BControl {
@@ -351,6 +353,7 @@ __device__ BState next_bstate(const BArgs &a, const BState &p, const unsigned lo
   s.slot = p.slot + 1 == a.ring ? 0 : p.slot + 1;
   s.pull_words = pull;
   s.push_words = live & ~pull;
+  if (pull) s.pulled = 1;
   s.phase = s.push_words ? BPH_TD : BPH_BU;
   // A small all-push frontier is pushed and settled in one launch.
   if (pull == 0 && a.fused > 0 && mf <= (unsigned long long)a.fused) s.phase = BPH_TDF;
@@ -1131,14 +1134,25 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, boo
 // Launch i runs as two kernels queued back to back: the bottom-up phase has
 // its own kernel (its mask registers would otherwise cap the occupancy of
 // every phase), the other phases share one. Both derive the same state for
-// launch i; the one whose phase it is does the work.
+// launch i; the one whose phase it is does the work. When the previous
+// search on the workspace never pulled (W <= 4), the host queues k_blevel
+// alone, with pulls = 1, and a pull runs there.
 template <int BS, int W>
-__global__ __launch_bounds__(BS) void k_blevel(BArgs a, long long i) {
+__global__ __launch_bounds__(BS) void k_blevel(BArgs a, long long i, int pulls) {
   __shared__ BSmem<BS> sm;
   begin_blaunch<BS, W>(a, sm, i);
   const BState &S = sm.S;
-  if (S.done || S.phase == BPH_BU) return;
+  if (S.done) return;
   const int slot = int(i % 3);
+  if (S.phase == BPH_BU) {
+    // With pulls set (W <= 4 only) the host queued no k_blevel_bu for this
+    // launch: the pull runs here, at this kernel's occupancy.
+    if (W <= 4 && pulls) {
+      bbu_phase<BS, W>(a, sm, S.level);
+      block_counts_flush(a, sm, slot);
+    }
+    return;
+  }
   switch (S.phase) {
     case BPH_TD: btd_phase<BS, W, false>(a, sm, S.level); break;
     case BPH_TDF:
@@ -1333,6 +1347,9 @@ struct BatchWorkspace {
   int expect_launches = 16;
   int *rows = nullptr;     // device rows of ent_bfs_batch (grown on demand)
   size_t rows_bytes = 0;
+  // The last search pulled at some level: queue k_blevel_bu beside every
+  // launch (otherwise, with W <= 4, k_blevel runs a pull itself).
+  bool pulled_last = true;
   // Every nonzero mask in the lv ring belongs to a vertex in its slot's
   // bitmap (what fused levels rely on). Holds after the rings are zeroed and
   // after a search that completed with fused levels allowed; a search that
@@ -1512,14 +1529,20 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   const int lgrid = c.grid_blocks > 0 ? c.grid_blocks : g->cus * std::max(1, c.blocks_per_cu > 0 ? c.blocks_per_cu : 8);
   long long i = 0;
   int kk = std::max(2, b->expect_launches);
+  // A search like the last one that never pulled queues one kernel per
+  // launch; a pull it does meet runs in k_blevel (W <= 4). W = 8 always
+  // queues the pull kernel, whose registers k_blevel lacks.
+  const bool pull_kernel = W > 4 || b->pulled_last;
+  int pulls = pull_kernel ? 0 : 1;
   // At most two launches per level (top-down, then pull or apply), one flush
   // per lcap + 1 levels and the output; levels <= n.
   const long long max_launches = 4LL * ((long long)g->n + 4);
   for (;;) {
     for (int j = 0; j < kk; ++j, ++i) {
+      void *gargs[] = {&a, &i, &pulls};
       void *args[] = {&a, &i};
-      HIP_TRY(hipLaunchKernel(kl.gen, dim3(lgrid), dim3(kBatchBS), args, 0, g->stream));
-      HIP_TRY(hipLaunchKernel(kl.bu, dim3(lgrid), dim3(kBatchBS), args, 0, g->stream));
+      HIP_TRY(hipLaunchKernel(kl.gen, dim3(lgrid), dim3(kBatchBS), gargs, 0, g->stream));
+      if (pull_kernel) HIP_TRY(hipLaunchKernel(kl.bu, dim3(lgrid), dim3(kBatchBS), args, 0, g->stream));
     }
     HIP_TRY(hipMemcpyAsync(b->h_ctl, b->ctl, sizeof(BState) * 2, hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
@@ -1529,6 +1552,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
     kk = std::min(2 * kk, 512);
   }
   b->expect_launches = std::min(fin->phases + 1, 512);
+  b->pulled_last = fin->pulled != 0;
   g->last_grid = lgrid;
   b->rings_clean = a.fused > 0 && !fin->error;
   return ENT_OK;

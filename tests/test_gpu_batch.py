@@ -401,3 +401,27 @@ def test_batch_fused_directed_topdown(oracle):
             for i, r in enumerate(roots):
                 assert np.array_equal(out[i], oracle.bfs(beg, adj, int(r))), (k, i, int(r))
             assert np.any(out[:, 3000:] > 0)
+
+
+def test_batch_pull_in_level_kernel(oracle, rmat16):
+    """After a search that never pulled, the next search on the workspace
+    queues one kernel per launch, and a pull it meets runs in that kernel
+    (W <= 4). A forced top-down batch, then two direction-optimizing batches
+    (the first pulls inside k_blevel, the second with the pull kernel
+    queued again), for W = 1 and W = 4: every row against the single-source
+    path and the first rows against the oracle."""
+    g, dg = rmat16
+    base = dg.config
+    try:
+        for k in (64, 256):
+            dg.batch_release()
+            roots = g.pick_roots_mixed(k, k // 2, 5000 + k)
+            dg.configure(direction=L.DIR_TOPDOWN)
+            st = _check_batch(oracle, g, dg, roots, f"rmat16/k{k}/topdown")
+            assert st["bu_level_mask"] == 0
+            dg.configure(direction=L.DIR_AUTO)
+            for j in range(2):
+                st = _check_batch(oracle, g, dg, roots, f"rmat16/k{k}/auto{j}")
+                assert st["bu_level_mask"] != 0 and st["words"] == k // 64
+    finally:
+        dg.set_config(base)
