@@ -84,7 +84,7 @@ def batch(args):
     try:
         dg.batch_reserve(k, 48)
         if args.warmup > 0:
-            dg.bfs_batch_device(roots[:64], out)        # warm-up
+            dg.bfs_batch_device(roots, out)             # warm-up
         stats = []
         for _ in range(max(1, args.steps if args.steps < 64 else 3)):
             stats.append(dg.bfs_batch_device(roots, out))

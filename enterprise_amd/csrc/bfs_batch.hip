@@ -1355,6 +1355,7 @@ struct BatchWorkspace {
   // after a search that completed with fused levels allowed; a search that
   // did not, or failed, may leave masks the bitmaps do not list.
   bool rings_clean = false;
+  int clean_words = 0;  // the mask width of the layout rings_clean describes
 };
 
 }  // namespace ent
@@ -1430,7 +1431,10 @@ BKernels blevel_for(int W) {
 
 int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) {
   BatchWorkspace *b = g->batch;
-  const int W = b->words;
+  // The search's own width: a workspace reserved for more sources holds the
+  // narrower layout (every array is n * W words per level slot or less), and
+  // a batch of 64 on a 512-source workspace costs what it costs on its own.
+  const int W = words_for(k);
   BArgs a{};
   a.off = g->off;
   a.adj = g->adj;
@@ -1484,8 +1488,9 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   init.munf = (unsigned long long)g->m;  // every vertex still needs sources
   HIP_TRY(hipMemcpyAsync(b->roots, roots, sizeof(int32_t) * size_t(k), hipMemcpyHostToDevice, g->stream));
   const int grid = g->cus * 8;
-  const bool was_clean = b->rings_clean;
+  const bool was_clean = b->rings_clean && b->clean_words == W;  // same layout
   b->rings_clean = false;  // until this search completes
+  b->clean_words = W;
   if (a.fused > 0 && !was_clean) {
     const long long nlv = a.nW * (long long)a.ring, njf = (long long)a.nwords * (long long)a.ring;
     hipLaunchKernelGGL(k_zero_u64, dim3(grid), dim3(256), 0, g->stream, b->lv, nlv);
@@ -1609,7 +1614,7 @@ int batch_locked(ent_graph *g, const int32_t *roots, int k, int32_t *depths_dev,
   if (st) {
     std::memset(st, 0, sizeof *st);
     st->k = k;
-    st->words = g->batch->words;
+    st->words = words_for(k);
     st->levels = fin.levels;
     st->phases = fin.phases;
     st->reached_edges = int64_t(edges);

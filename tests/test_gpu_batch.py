@@ -425,3 +425,25 @@ def test_batch_pull_in_level_kernel(oracle, rmat16):
                 assert st["bu_level_mask"] != 0 and st["words"] == k // 64
     finally:
         dg.set_config(base)
+
+
+def test_batch_narrow_on_wide_workspace(oracle, rmat16):
+    """A workspace reserved for 512 sources runs each batch at the width its
+    own source count needs: 64, 200, 512 and 64 sources again, in both launch
+    forms, each row against the single-source path and the first rows
+    against the oracle; the fused 64-source search after the 512-source one
+    starts from rings in another layout."""
+    g, dg = rmat16
+    base = dg.config
+    try:
+        for launch in (L.LAUNCH_STREAM, L.LAUNCH_PERSISTENT):
+            dg.configure(launch=launch)
+            dg.batch_release()
+            dg.batch_reserve(512, 12)
+            for j, k in enumerate((64, 200, 512, 64)):
+                roots = g.pick_roots_mixed(k, k // 2, 6000 + 10 * j + k)
+                st = _check_batch(oracle, g, dg, roots, f"rmat16/wide/{launch}/{k}")
+                assert st["words"] == (1 if k <= 64 else 4 if k <= 256 else 8)
+    finally:
+        dg.set_config(base)
+        dg.batch_release()
