@@ -32,7 +32,9 @@ extern "C" {
 
 typedef struct ent_batch_stats {
   int32_t k;                /* sources */
-  int32_t words;            /* u64 masks per vertex */
+  int32_t words;            /* u64 masks per vertex in this search: k / 64
+                               rounded up to 1, 2, 4 or 8, whatever width the
+                               workspace was reserved for */
   int32_t levels;           /* joint levels (max over sources of depth + 1) */
   int32_t phases;           /* kernel phases run */
   int64_t reached_edges;    /* sum over sources of M_c(s) */
@@ -55,7 +57,8 @@ typedef struct ent_batch_stats {
 
 /* Reserves the workspace for up to k sources, keeping max_levels joint
  * levels between flushes (1..250; 0 = keep the current reservation or the
- * default of 32 levels). */
+ * default of 32 levels). A search with fewer sources runs in it at its own
+ * mask width. */
 int ent_batch_reserve(ent_graph *g, int32_t k, int32_t max_levels);
 
 /* Frees the multi-source workspace (the graph stays resident). */
