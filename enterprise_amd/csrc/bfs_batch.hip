@@ -145,7 +145,7 @@ struct BArgs {
   int out16;  // out is 16-byte aligned and n % 4 == 0: rows in 16-byte stores
   // Largest joint-frontier degree sum expanded as a fused level (0 = never;
   // W <= 4 only). When nonzero, every level also empties the ring slot two
-  // levels ahead (see prep_slot).
+  // levels ahead (see prep_word).
   long long fused;
 };
 
@@ -442,7 +442,7 @@ struct Claims {
 // One edge of the push to u. Plain: OR the bits of f that u has not seen into
 // lv[L + 1][u], fire and forget. CLAIM (a fused level): the bits also go into
 // seen[u] at once, and the OR into lv[L + 1][u] returns the word's old value.
-// That word was zero before the level began (prep_slot), so the edge that
+// That word was zero before the level began (prep_word), so the edge that
 // finds it zero is the first to reach u in that word: it counts u's word, and
 // the first word to claim u also sets u's bit in the new joint frontier jn.
 template <int W, bool CLAIM>
@@ -492,34 +492,30 @@ __device__ __forceinline__ void push_range(const BArgs &a, unsigned long long *d
   for (long long i = s + t; i < e; i += stride) push_edge<W, CLAIM>(a, dst, a.adj[i], f, jn, cl);
 }
 
-// Empties ring slot `slot` for the level that will next use it: clears the
-// mask of every vertex in the slot's old joint frontier, then the bitmap.
-// Every nonzero mask of a slot belongs to a vertex in that slot's bitmap (a
-// push reaches only vertices that join the new frontier, a pull stores only
-// new bits), so the slot is then zero throughout, as a fused level's claims
-// require. Thread per bitmap word.
-template <int BS, int W>
-__device__ void prep_slot(const BArgs &a, int slot) {
-  unsigned long long *lz = level_ptr(a, slot);
-  unsigned long long *jz = jf_ptr(a, slot);
-  const long long nt = (long long)gridDim.x * BS;
-  for (long long wd = (long long)blockIdx.x * BS + threadIdx.x; wd < a.nwords; wd += nt) {
-    unsigned long long m = jz[wd];
-    if (!m) continue;
-    jz[wd] = 0;
-    while (m) {
-      const long long v = wd * 64 + (__ffsll((long long)m) - 1);
-      m &= m - 1;
+// Empties word wd of a ring slot (masks lz, bitmap jz) for the level that
+// will next use the slot: clears the mask of every vertex of the word in
+// the slot's old joint frontier, then the word. Every nonzero mask of a slot
+// belongs to a vertex in that slot's bitmap (a push reaches only vertices
+// that join the new frontier, a pull stores only new bits), so once every
+// word is emptied the slot is zero throughout, as a fused level's claims
+// require. Every level empties the slot of level L + 2 this way (the fused
+// level beside its sweep, APPLY and the pull in theirs).
+template <int W>
+__device__ __forceinline__ void prep_word(unsigned long long *lz, unsigned long long *jz, long long wd) {
+  unsigned long long m = jz[wd];
+  if (!m) return;
+  jz[wd] = 0;
+  for (; m; m &= m - 1) {
+    const long long v = wd * 64 + (__ffsll((long long)m) - 1);
 #pragma unroll
-      for (int w = 0; w < W; ++w) lz[v * W + w] = 0;
-    }
+    for (int w = 0; w < W; ++w) lz[v * W + w] = 0;
   }
 }
 
 // Top-down level L. Plain (TD): pushes only; APPLY or the pull settles the
 // new bits. CLAIM (TDF, a fused level): pushes with claims, so the new
 // frontier's bitmap, seen and the counts are complete when the launch ends,
-// and empties the slot of level L + 2 (prep_slot) as APPLY and the pull do.
+// and empties the slot of level L + 2 (prep_word) as APPLY and the pull do.
 template <int BS, int W, bool CLAIM>
 __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const unsigned long long *fs = level_ptr(a, ring_slot(a, sm.S, 0));
@@ -554,9 +550,14 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   const int CW = CLAIM ? kWave : a.chunk_words;
   const long long nchunks = ((long long)a.nwords + CW - 1) / CW;
   const long long nwv = (long long)gridDim.x * (BS / kWave);
+  unsigned long long *lz = level_ptr(a, ring_slot(a, sm.S, 2));
+  unsigned long long *jz = jf_ptr(a, ring_slot(a, sm.S, 2));
   for (long long ch = (long long)blockIdx.x * (BS / kWave) + wid; ch < nchunks; ch += nwv) {
     const long long w = ch * CW + lane;
     const unsigned long long word = (lane < CW && w < a.nwords) ? jf[w] : 0ull;
+    // The slot of level L + 2, word w (a fused level's chunks cover every
+    // word once, a word per lane), emptied beside the sweep.
+    if (CLAIM && w < a.nwords) prep_word<W>(lz, jz, w);
     if (__ballot(word != 0) == 0) continue;
     const int cnt = __popcll(word);
     const int incl = wave_incl_scan(cnt);
@@ -597,7 +598,6 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     wave_sync();
   }
   if (CLAIM) {
-    prep_slot<BS, W>(a, ring_slot(a, sm.S, 2));
     WordSums<W> mfw;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -610,7 +610,7 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   }
 }
 
-// prep_slot for one bitmap word inside a sweep with a lane per vertex: the
+// prep_word inside a sweep with a lane per vertex: the
 // masks of the word's old members (bits of oj, wave-uniform), then the word.
 template <int W>
 __device__ __forceinline__ void clear_old_members(unsigned long long *lz, unsigned long long *jz, long long wd,
@@ -672,7 +672,7 @@ __device__ void bapply_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   unsigned long long *jn = jf_ptr(a, ring_slot(a, sm.S, 1));
   unsigned long long *clr = level_ptr(a, ring_slot(a, sm.S, 2));  // lv[L + 2]
   unsigned long long *pj = jf_ptr(a, ring_slot(a, sm.S, 2));
-  const bool prep = W <= 4 && a.fused > 0;  // empty the slot of level L + 2 (prep_slot)
+  const bool prep = W <= 4 && a.fused > 0;  // empty the slot of level L + 2 (prep_word)
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
@@ -774,7 +774,7 @@ __device__ void bbu_phase_h(const BArgs &a, BSmem<BS> &sm, int L) {
   unsigned long long *jn = jf_ptr(a, ring_slot(a, sm.S, 1));
   unsigned long long *clr = level_ptr(a, ring_slot(a, sm.S, 2));  // lv[L + 2]
   unsigned long long *pj = jf_ptr(a, ring_slot(a, sm.S, 2));
-  const bool prep = W <= 4 && a.fused > 0;  // empty the slot of level L + 2 (prep_slot)
+  const bool prep = W <= 4 && a.fused > 0;  // empty the slot of level L + 2 (prep_word)
   const int lane = lane_id();
   const long long nwv = (long long)gridDim.x * (BS / kWave);
   unsigned long long jw = 0, mf = 0, alg = 0, munf = 0;
@@ -1228,7 +1228,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(bpersist_wav
 // Zeroes seen, the level-0 joint frontier, lv[0] at the roots (the only
 // vertices where it is read) and lv[1] where level 0 can push; clears the
 // counters and the grid barrier; sets the pseudo-level -1 state. With fused
-// levels, also empties the slots of levels 0 and 1 whole (prep_slot; the
+// levels, also empties the slots of levels 0 and 1 whole (prep_word; the
 // level-1 masks are cleared at every vertex a push can reach).
 __global__ void k_binit_zero(BArgs a, const int *roots) {
   const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
