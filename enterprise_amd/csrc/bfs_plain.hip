@@ -952,13 +952,14 @@ __device__ __forceinline__ void begin_launch(const Args &a, Smem<BS> &sm, long l
 }
 
 // depth = -1 (root 0), visited = isolated vertices, padding bits and the root,
-// front[0] = {root}, marks cleared. A vertex without edges is no one's
-// neighbour on a symmetric adjacency, the only kind bottom-up levels search
-// exactly; a forced top-down search, exact on any adjacency, marks only the
-// padding bits, since a vertex without out-edges may still have in-edges. The state before launch 0 is a finished
-// pseudo-level -1 whose counts (cnt[2]) describe frontier 0 = {root}.
+// front[0] = {root}, marks cleared (clear_marks). A vertex without edges is
+// no one's neighbour on a symmetric adjacency, the only kind bottom-up levels
+// search exactly; a forced top-down search, exact on any adjacency, marks
+// only the padding bits, since a vertex without out-edges may still have
+// in-edges. The state before launch 0 is a finished pseudo-level -1 whose
+// counts (cnt[2]) describe frontier 0 = {root}.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu, int search) {
+__global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu, int search, int clear_marks) {
   const long long gt = (long long)blockIdx.x * BS + threadIdx.x;
   const long long nt = (long long)gridDim.x * BS;
   // Depths in 16-byte stores from the array's first 16-byte boundary (a
@@ -977,10 +978,14 @@ __global__ __launch_bounds__(BS) void k_init(Args a, int root, int first_bu, int
     const long long v = gt < h ? gt : t0 + (gt - h);
     a.depth[v] = v == root ? 0 : -1;
   }
-  // Marks (the library's own allocation, 256-byte aligned) in 16-byte stores.
-  const long long nm = (long long)a.n >> 4;
-  for (long long q = gt; q < nm; q += nt) reinterpret_cast<uint4 *>(a.mark)[q] = make_uint4(0u, 0u, 0u, 0u);
-  if (gt < (long long)a.n - (nm << 4)) a.mark[(nm << 4) + gt] = 0;
+  // Marks (the library's own allocation, 256-byte aligned) in 16-byte
+  // stores, unless the last search on the handle completed and so left them
+  // all zero.
+  if (clear_marks) {
+    const long long nm = (long long)a.n >> 4;
+    for (long long q = gt; q < nm; q += nt) reinterpret_cast<uint4 *>(a.mark)[q] = make_uint4(0u, 0u, 0u, 0u);
+    if (gt < (long long)a.n - (nm << 4)) a.mark[(nm << 4) + gt] = 0;
+  }
   for (long long w = gt; w < a.nwords; w += nt) {
     const unsigned long long rb = (w == (root >> 6)) ? 1ull << (root & 63) : 0ull;
     const unsigned long long pad = (w == a.nwords - 1 && (a.n & 63)) ? ~0ull << (a.n & 63) : 0ull;
@@ -1454,8 +1459,10 @@ int run_search(ent_graph *g, int root, int *depth_dev, State *fin) {
   int first_bu = cfg.direction == ENT_DIR_BOTTOMUP ? 1 : 0;
   if (++g->search_seq == 0) g->search_seq = 1;  // 0 marks the zeroed mirror
   int search = int(g->search_seq);
+  int clear_marks = g->marks_clean ? 0 : 1;
+  g->marks_clean = false;  // until this search completes
   {
-    void *args[] = {&a, &root, &first_bu, &search};
+    void *args[] = {&a, &root, &first_bu, &search, &clear_marks};
     const int grid = int(std::min<int64_t>(int64_t(g->cus) * 8, (g->n + bs - 1) / bs + 1));
     HIP_TRY(launch(ks.init, std::max(1, grid), bs, args, g->stream));
   }
@@ -1810,6 +1817,7 @@ static int search_locked(ent_graph *g, int root, int *depth, ent_stats *st,
   if (c.error)
     return set_error(ENT_EINTERNAL, "device reported error flags 0x%x (2 level guard, 4 barrier timeout)", c.error);
   if (!c.done) return set_error(ENT_EINTERNAL, "search ended without completion");
+  g->marks_clean = true;
   // The last counted launch finds the frontier empty and records nothing.
   if (g->cfg.count_examined) g->trace_launches = c.phases - 1;
   if (st) {

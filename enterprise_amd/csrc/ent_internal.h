@@ -73,6 +73,10 @@ struct ent_graph {
   unsigned long long *visited = nullptr;
   unsigned long long *front[2] = {nullptr, nullptr};
   unsigned char *mark = nullptr;
+  // Every mark is zero: the last search on the handle completed (a top-down
+  // level marks only unvisited vertices, and the inspection that always
+  // follows it clears every mark it finds). False until the first search.
+  bool marks_clean = false;
   int *qbuf[2] = {nullptr, nullptr};  // striped frontier queues (kQStripes x qcap_s)
   int qcap_s = 0;
   ent::Control *ctl = nullptr;
