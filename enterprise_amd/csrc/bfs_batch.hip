@@ -143,6 +143,7 @@ struct BArgs {
   int probe;
   int chunk_words;
   int out16;  // out is 16-byte aligned and n % 4 == 0: rows in 16-byte stores
+  const int *rowmap;  // output row of the source at mask bit p (sources are placed by root degree)
   // Largest joint-frontier degree sum expanded as a fused level (0 = never;
   // W <= 4 only). When nonzero, every level also empties the ring slot two
   // levels ahead (see prep_word).
@@ -205,8 +206,13 @@ __device__ __forceinline__ bool receives(const BArgs &a, long long deg) {
 template <int BS>
 struct BSmem {
   static constexpr int NW = BS / kWave;
-  unsigned long long cand[NW][kWave];
-  int excl[NW][kWave];
+  union {
+    struct {
+      unsigned long long cand[NW][kWave];
+      int excl[NW][kWave];
+    } td;                              // top-down frontier staging
+    int rows[ENT_BATCH_MAX_SOURCES];   // output: row of the source at each mask bit
+  };
   unsigned char d8[NW][kWave * 68];  // output staging: 64 lanes x 64 sources (+4 pad)
   unsigned long long c[C_COUNT];
   long long seg;  // output: the segment the block claimed
@@ -543,8 +549,8 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
     push_range<W, CLAIM>(a, dst, a.off[v], a.off[v + 1], threadIdx.x, BS, f, jn, cl);
   }
   const int wid = threadIdx.x / kWave, lane = lane_id();
-  unsigned long long *wc = sm.cand[wid];
-  int *we = sm.excl[wid];
+  unsigned long long *wc = sm.td.cand[wid];
+  int *we = sm.td.excl[wid];
   // A fused level's frontier is sparse: chunks of 64 bitmap words (4,096
   // vertices) keep a wavefront's lanes busy and its chunks per level few.
   const int CW = CLAIM ? kWave : a.chunk_words;
@@ -1012,6 +1018,9 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, boo
     const int t = s0 + (L - lo);
     return t >= a.ring ? t - a.ring : t;
   };
+  // Output row of each mask bit (the first block barrier of the loop below
+  // orders these stores before the reads).
+  for (int p = threadIdx.x; p < 64 * W; p += BS) sm.rows[p] = p < a.k ? a.rowmap[p] : 0;
   // Segments per claim and this wavefront's segment and first group.
   constexpr int SPC = W == 1 ? NWV : 1;
   const int wseg = W == 1 ? wid : 0;
@@ -1105,7 +1114,7 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, boo
             typedef int v4i __attribute__((ext_vector_type(4)));
             const unsigned char d0 = d8[(4 * q + 0) * 68 + b], d1 = d8[(4 * q + 1) * 68 + b];
             const unsigned char d2 = d8[(4 * q + 2) * 68 + b], d3 = d8[(4 * q + 3) * 68 + b];
-            int *dst = a.out + (long long)(64 * g + b) * a.n + v0;
+            int *dst = a.out + (long long)sm.rows[64 * g + b] * a.n + v0;
             v4i x;
             x.x = d0 == 0xFF ? -1 : int(d0);
             x.y = d1 == 0xFF ? -1 : int(d1);
@@ -1117,7 +1126,7 @@ __device__ void boutput_phase(const BArgs &a, BSmem<BS> &sm, int lo, int hi, boo
       } else if (valid) {
         for (int b = 0; b < nb; ++b) {
           const unsigned char d = d8[lane * 68 + b];
-          int *dst = a.out + (long long)(64 * g + b) * a.n + v;
+          int *dst = a.out + (long long)sm.rows[64 * g + b] * a.n + v;
           if (lo == 0) *dst = d == 0xFF ? -1 : int(d);
           else if (d != 0xFF) *dst = lo + int(d);
         }
@@ -1340,7 +1349,7 @@ struct BatchWorkspace {
   unsigned long long *seen = nullptr;
   unsigned long long *lv = nullptr;
   unsigned long long *jf = nullptr;  // lcap + 3 bitmaps
-  int *roots = nullptr;
+  int *roots = nullptr;  // roots by mask bit, then the output row of each bit (2 x 512)
   BControl *ctl = nullptr;
   BControl *h_ctl = nullptr;
   size_t bytes = 0;
@@ -1372,7 +1381,7 @@ int words_for(int k) { return k <= 64 ? 1 : k <= 128 ? 2 : k <= 256 ? 4 : 8; }
 size_t ws_bytes(int64_t n, int W, int lcap) {
   const size_t nW = size_t(n) * size_t(W);
   const size_t nwords = size_t((n + 63) / 64);
-  return 8 * nW + (8 * nW + 8 * nwords) * size_t(lcap + 3) + sizeof(BControl) + 4 * ENT_BATCH_MAX_SOURCES;
+  return 8 * nW + (8 * nW + 8 * nwords) * size_t(lcap + 3) + sizeof(BControl) + 8 * ENT_BATCH_MAX_SOURCES;
 }
 
 int reserve(ent_graph *g, int k, int lcap) {
@@ -1396,7 +1405,7 @@ int reserve(ent_graph *g, int k, int lcap) {
   if ((rc = dev_alloc(g, reinterpret_cast<void **>(&b->seen), 8 * nW)) ||
       (rc = dev_alloc(g, reinterpret_cast<void **>(&b->lv), 8 * nW * size_t(lcap + 3))) ||
       (rc = dev_alloc(g, reinterpret_cast<void **>(&b->jf), 8 * size_t(g->nwords) * size_t(lcap + 3))) ||
-      (rc = dev_alloc(g, reinterpret_cast<void **>(&b->roots), 4 * ENT_BATCH_MAX_SOURCES)) ||
+      (rc = dev_alloc(g, reinterpret_cast<void **>(&b->roots), 8 * ENT_BATCH_MAX_SOURCES)) ||
       (rc = dev_alloc(g, reinterpret_cast<void **>(&b->ctl), sizeof(BControl)))) {
     batch_release(g);
     return rc;
@@ -1466,12 +1475,34 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   a.chunk_words = c.bu_chunk_words;
   a.out16 = ((reinterpret_cast<uintptr_t>(out) & 15) == 0 && (g->n & 3) == 0) ? 1 : 0;
   a.fused = (W <= 4 && c.batch_fused > 0) ? std::max<long long>(1, (long long)g->n / c.batch_fused) : 0;
+  // Sources are placed at mask bits in descending order of root degree
+  // (ties by vertex id), so each 64-source word holds roots whose frontiers
+  // grow alike and the per-word direction can push the lagging words while it
+  // pulls the others, whatever order the caller gave; the same set of roots
+  // always gets the same words, and so the same per-word counts.
+  // placed[0 .. k) are the roots by bit, placed[512 + p] the caller's index
+  // of bit p (its output row).
+  std::vector<int32_t> placed(2 * ENT_BATCH_MAX_SOURCES, 0);
+  {
+    std::vector<int> order(static_cast<size_t>(k));
+    for (int s = 0; s < k; ++s) order[size_t(s)] = s;
+    const int32_t *deg = g->store->h_deg.data();
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      const int32_t dx = deg[size_t(roots[x])], dy = deg[size_t(roots[y])];
+      return dx != dy ? dx > dy : roots[x] < roots[y];
+    });
+    for (int p = 0; p < k; ++p) {
+      placed[size_t(p)] = roots[order[size_t(p)]];
+      placed[size_t(ENT_BATCH_MAX_SOURCES + p)] = order[size_t(p)];
+    }
+  }
+  a.rowmap = b->roots + ENT_BATCH_MAX_SOURCES;
   // Level-0 counts on the host: distinct (vertex, word) pairs of the roots.
   std::vector<std::pair<int32_t, int>> vw;
   std::vector<int32_t> vs;
-  for (int s = 0; s < k; ++s) {
-    vw.push_back({roots[s], s >> 6});
-    vs.push_back(roots[s]);
+  for (int p = 0; p < k; ++p) {
+    vw.push_back({placed[size_t(p)], p >> 6});
+    vs.push_back(placed[size_t(p)]);
   }
   std::sort(vw.begin(), vw.end());
   vw.erase(std::unique(vw.begin(), vw.end()), vw.end());
@@ -1486,7 +1517,7 @@ int run_batch(ent_graph *g, const int32_t *roots, int k, int *out, BState *fin) 
   }
   init.jw = vw.size();
   init.munf = (unsigned long long)g->m;  // every vertex still needs sources
-  HIP_TRY(hipMemcpyAsync(b->roots, roots, sizeof(int32_t) * size_t(k), hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(b->roots, placed.data(), sizeof(int32_t) * placed.size(), hipMemcpyHostToDevice, g->stream));
   const int grid = g->cus * 8;
   const bool was_clean = b->rings_clean && b->clean_words == W;  // same layout
   b->rings_clean = false;  // until this search completes
@@ -1573,7 +1604,7 @@ void batch_release(ent_graph *g) {
   dev_free(g, b->seen, 8 * nW);
   dev_free(g, b->lv, 8 * nW * size_t(b->lcap + 3));
   dev_free(g, b->jf, 8 * size_t(g->nwords) * size_t(b->lcap + 3));
-  dev_free(g, b->roots, 4 * ENT_BATCH_MAX_SOURCES);
+  dev_free(g, b->roots, 8 * ENT_BATCH_MAX_SOURCES);
   dev_free(g, b->ctl, sizeof(BControl));
   dev_free(g, b->rows, b->rows_bytes);
   if (b->h_ctl) ignore(hipHostFree(b->h_ctl));

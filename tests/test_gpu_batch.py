@@ -447,3 +447,31 @@ def test_batch_narrow_on_wide_workspace(oracle, rmat16):
     finally:
         dg.set_config(base)
         dg.batch_release()
+
+
+def test_batch_root_order(oracle, rmat16):
+    """Sources are placed in the 64-source words by root degree, so the same
+    roots in another order give the same rows, permuted, and the same
+    counts: 512 mixed roots (256 hubs, 256 of degree 1..4) in the picker's
+    order, shuffled, and reversed, in both launch forms, every row against
+    the single-source path."""
+    g, dg = rmat16
+    roots = g.pick_roots_mixed(512, 256, 7100)
+    rng = np.random.default_rng(7101)
+    base = dg.config
+    try:
+        for launch in (L.LAUNCH_STREAM, L.LAUNCH_PERSISTENT):
+            dg.configure(launch=launch)
+            ref, ref_st = dg.bfs_batch(roots)
+            for name, perm in (("shuffled", rng.permutation(512)), ("reversed", np.arange(511, -1, -1))):
+                _say(f"[gpu] batch root order {name} {launch}")
+                out, st = dg.bfs_batch(roots[perm])
+                assert np.array_equal(out, ref[perm]), name
+                for key in ("levels", "reached_vertices", "reached_edges", "alg_bytes"):
+                    assert st[key] == ref_st[key], (name, key)
+            for i in (0, 255, 256, 511):
+                want, _ = dg.bfs(int(roots[i]))
+                assert np.array_equal(ref[i], want), i
+            assert np.array_equal(ref[0], oracle.bfs(g.beg_pos, g.adj, int(roots[0])))
+    finally:
+        dg.set_config(base)
