@@ -70,7 +70,10 @@ int64_t ent_batch_workspace_bytes(const ent_graph *g, int32_t k,
 
 /* Multi-source BFS from roots[0..k-1] (1 <= k <= 512, roots may repeat),
  * writing k * n int32 depths to device memory depths_dev (4-byte aligned;
- * rows go out in 16-byte stores when it is 16-byte aligned and n % 4 == 0). */
+ * rows go out in 16-byte stores when it is 16-byte aligned and n % 4 == 0).
+ * Row i is root i's depth array. Internally the sources are grouped into
+ * 64-source words by root degree, so the time and the counts do not depend
+ * on the order of roots. */
 int ent_bfs_batch_device(ent_graph *g, const int32_t *roots, int32_t k,
                          int32_t *depths_dev, ent_batch_stats *stats);
 
