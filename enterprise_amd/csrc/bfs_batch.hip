@@ -616,8 +616,8 @@ __device__ void btd_phase(const BArgs &a, BSmem<BS> &sm, int L) {
   }
 }
 
-// prep_word inside a sweep with a lane per vertex: the
-// masks of the word's old members (bits of oj, wave-uniform), then the word.
+// prep_word inside a sweep with a lane per vertex: the masks of the word's
+// old members (bits of oj, wave-uniform), then the word.
 template <int W>
 __device__ __forceinline__ void clear_old_members(unsigned long long *lz, unsigned long long *jz, long long wd,
                                                   unsigned long long oj, int lane) {
@@ -980,7 +980,8 @@ __device__ __forceinline__ void replay_lanes(const BArgs &a, unsigned char *d8, 
     } else {
 #pragma unroll
       for (int t = 0; t < kLv; ++t)
-        for (unsigned long long b = mk[t]; b; b &= b - 1) d8[lane * 68 + (__ffsll((long long)b) - 1)] = (unsigned char)lvl[t];
+        for (unsigned long long b = mk[t]; b; b &= b - 1)
+          d8[lane * 68 + (__ffsll((long long)b) - 1)] = (unsigned char)lvl[t];
     }
   }
 }
